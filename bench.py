@@ -1,0 +1,223 @@
+"""Benchmark: HiCOO MTTKRP on one MI355X (configs[1] workload).
+
+Workload: order-3 tensor 2^20 x 2^20 x 2^16, 100M unique nonzeros, each
+mode's index Zipf(1.1) over its dimension and randomly relabelled, values
+U(0,1); HiCOO with sb=7, sk=14, sc=14; rank 32 factors U(0,1).  A step is
+one MTTKRP per mode (modes 0, 1, 2) on the device-resident tensor, i.e. one
+pass of the hot path over the whole tensor.  Inputs are in HBM before the
+timed region starts.
+
+Printed JSON (one line): value = milliseconds per step (all three modes,
+FP32 factors, lower is better); the roofline of the HiCOO kernel with
+algorithmic bytes per launch
+    B = nnz*(N*1 + 4) + nb*(N*4 + 8) + sum_m ndims[m]*R*4
+(the reference's convention, benchmark/tensor/mttkrp.c:304-307, applied to
+the HiCOO arrays) divided by the measured average duration of one mode's
+MTTKRP (HIP events on the launch stream; this includes the output zero
+fill), against 8 TB/s; the BF16-factor form's time; and the oracle's
+OpenMP HiCOO MTTKRP (the reference's superblock-parallel scheme) timed on a
+bounded sample of the same tensor.
+
+    python bench.py [--steps K] [--warmup W] [--nnz NNZ] [--no-cpu]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--nnz", type=float, default=100e6)
+    p.add_argument("--rank", type=int, default=32)
+    p.add_argument("--seed", type=int, default=42)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-bf16", action="store_true")
+    p.add_argument("--cpu-sample-nnz", type=float, default=8e6)
+    return p.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(h, ndims, facs_np, R, sample_nnz, nthreads):
+    """The oracle's OpenMP HiCOO MTTKRP (superblock-parallel with per-element
+    atomics, src/sptensor/hicoo/mttkrp_omp.c:400-467) on the leading
+    superblocks holding about sample_nnz nonzeros; one warm-up and two timed
+    calls per mode.  Returns ms per full three-mode step, extrapolated per
+    nonzero."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    a = h.address
+    N = len(ndims)
+    nk_total = int(a.kptr.len) - 1
+    kptr = np.ctypeslib.as_array(a.kptr.data, shape=(nk_total + 1,))
+    bptr_all = np.ctypeslib.as_array(a.bptr.data, shape=(int(a.bptr.len),))
+    # smallest prefix of superblocks with >= sample_nnz nonzeros
+    k_end = int(np.searchsorted(bptr_all[kptr], sample_nnz))
+    k_end = max(1, min(k_end, nk_total))
+    b_end = int(kptr[k_end])
+    z_end = int(bptr_all[b_end])
+    sub = {
+        "kptr": kptr[:k_end + 1].copy(),
+        "bptr": bptr_all[:b_end + 1].copy(),
+        "binds": np.stack([np.ctypeslib.as_array(a.binds[m].data, shape=(b_end,)) for m in range(N)]),
+        "einds": np.stack([np.ctypeslib.as_array(a.einds[m].data, shape=(z_end,)) for m in range(N)]),
+        "values": np.ctypeslib.as_array(a.values.data, shape=(z_end,)).copy(),
+    }
+    total = 0.0
+    for mode in range(N):
+        O.mttkrp_hicoo(sub, int(a.sb_bits), facs_np, mode, kind="omp", nthreads=nthreads)
+        t0 = time.perf_counter()
+        for _ in range(2):
+            O.mttkrp_hicoo(sub, int(a.sb_bits), facs_np, mode, kind="omp", nthreads=nthreads)
+        total += (time.perf_counter() - t0) / 2
+    per_nnz = total / z_end
+    return per_nnz * int(a.nnz) * 1e3, z_end, k_end
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from hiparti_amd import COOTensor, DenseMatrix, lib
+    from hiparti_amd.device import DevHiCOO
+
+    assert torch.cuda.is_available(), "no GPU"
+    ndims = [1 << 20, 1 << 20, 1 << 16]
+    nnz = int(args.nnz)
+    R = args.rank
+    N = 3
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.time()
+    X = COOTensor.synthetic(ndims, nnz, dist=[1, 1, 1], zipf_s=[1.1] * 3, permute=[1, 1, 1], seed=args.seed,
+                            nthreads=nthreads)
+    log("generated %d nnz in %.1f s" % (X.nnz(), time.time() - t0))
+    t0 = time.time()
+    H = X.toHiCOO(7, 14, 14, nthreads=nthreads)
+    nb = H.nblocks()
+    log("HiCOO conversion: nb=%d in %.1f s" % (nb, time.time() - t0))
+    X.free()
+    t0 = time.time()
+    D = DevHiCOO(H)
+    torch.cuda.synchronize()
+    log("upload %.2f GB in %.1f s" % (D.bytes / 1e9, time.time() - t0))
+
+    facs_np = []
+    dfacs = []
+    for m, d in enumerate(ndims):
+        dm = DenseMatrix(d, R)
+        dm.uniform(args.seed * 100 + m)
+        f = dm.numpy()
+        facs_np.append(f)
+        dfacs.append(torch.from_numpy(f).cuda())
+    outs = [torch.empty((d, R), dtype=torch.float32, device="cuda") for d in ndims]
+    bfacs = [f.to(torch.bfloat16) for f in dfacs]
+
+    def step(bf16=False, events=None):
+        for mode in range(N):
+            if events is not None:
+                events[mode][0].record()
+            if bf16:
+                D.mttkrp_bf16(bfacs, mode, R=R, out=outs[mode])
+            else:
+                D.mttkrp(dfacs, mode, R=R, out=outs[mode])
+            if events is not None:
+                events[mode][1].record()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(N)]
+          for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(events=ev[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per_mode = [float(np.mean([ev[k][m][0].elapsed_time(ev[k][m][1]) for k in range(args.steps)]))
+                for m in range(N)]
+    ms_per_step = wall * 1e3 / args.steps
+
+    bf16_ms = None
+    bf16_per_mode = None
+    if not args.no_bf16:
+        for _ in range(args.warmup):
+            step(bf16=True)
+        torch.cuda.synchronize()
+        evb = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(N)]
+               for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            step(bf16=True, events=evb[k])
+        torch.cuda.synchronize()
+        bf16_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        bf16_per_mode = [float(np.mean([evb[k][m][0].elapsed_time(evb[k][m][1]) for k in range(args.steps)]))
+                         for m in range(N)]
+
+    # Algorithmic bytes per single-mode MTTKRP (see module docstring).
+    fac_bytes = sum(d * R * 4 for d in ndims)
+    alg_bytes = nnz * (N * 1 + 4) + nb * (N * 4 + 8) + fac_bytes
+    avg_ms = float(np.mean(per_mode))
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "hicoo_mttkrp_kernel<3,4,8,float> (+ output zero fill)",
+                "alg_bytes_per_launch": int(alg_bytes)}
+    extra = {"per_mode_ms": [round(x, 4) for x in per_mode], "nblocks": nb, "nnz": nnz, "rank": R}
+    if bf16_ms is not None:
+        bf_bytes = nnz * (N * 1 + 4) + nb * (N * 4 + 8) + sum(d * R * 2 for d in ndims)
+        extra["bf16_ms_per_step"] = round(bf16_ms, 4)
+        extra["bf16_per_mode_ms"] = [round(x, 4) for x in bf16_per_mode]
+        extra["bf16_hbm_frac"] = round(bf_bytes / (np.mean(bf16_per_mode) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+
+    cpu = None
+    if not args.no_cpu:
+        t0 = time.time()
+        cpu_ms, z_s, k_s = cpu_baseline(H, ndims, facs_np, R, args.cpu_sample_nnz, nthreads)
+        cpu = {"value": round(cpu_ms, 1), "unit": "ms per step (3-mode HiCOO MTTKRP, extrapolated per nonzero)",
+               "cores": nthreads, "kind": "port",
+               "sample": "oracle OpenMP HiCOO MTTKRP (tk=%d, tb=1) on the first %d superblocks = %d nonzeros of "
+                         "the same tensor, all 3 modes, 1 warm-up + 2 timed calls per mode" % (nthreads, k_s, z_s)}
+        log("cpu baseline %.1f s" % (time.time() - t0))
+
+    line = {
+        "metric": "HiCOO MTTKRP ms per 3-mode sweep (fp32) & % HBM peak",
+        "value": round(ms_per_step, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, seed %d)" % args.seed,
+        "config": {"workload": "HiCOO MTTKRP all 3 modes, 2^20 x 2^20 x 2^16, %d nnz, Zipf(1.1) relabelled, "
+                               "sb=7 sk=14 sc=14, R=%d" % (nnz, R), "nnz": nnz, "rank": R},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

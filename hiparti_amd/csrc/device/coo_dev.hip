@@ -1,0 +1,250 @@
+// COO tensors resident in HBM and the COO MTTKRP entry points.
+//
+// The uploaded tensor keeps its storage order (SoA: one u32 array per mode
+// plus the values).  For each target mode a copy sorted by that mode's index
+// is built on the device on first use (stable radix sort of (index, position)
+// pairs with rocPRIM, then a gather), so that equal output rows are adjacent
+// and the kernel's segmented reduction collapses them into one atomic per
+// (chunk, row) run.
+#include <atomic>
+#include <mutex>
+#include <unordered_map>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "../../../include/hiparti_dev.h"
+#include "dropin.hpp"
+#include "launch.hpp"
+
+struct ptiDevCOO {
+    uint32_t nmodes = 0;
+    uint64_t nnz = 0;
+    ptiIndex ndims[8] = {0};
+    pti::dev::Buffer inds[8], vals;           // storage order
+    pti::dev::Buffer sinds[8][8], svals[8];   // [target mode][mode] sorted copies
+    bool ready[8] = {false};
+    std::mutex mu;
+};
+
+namespace pti {
+namespace dev {
+namespace {
+
+std::mutex g_coo_mu;
+std::unordered_map<const void *, ptiDevCOO *> g_coo_reg;
+
+__global__ void __launch_bounds__(256) iota_kernel(uint32_t *p, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = (uint32_t)i;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) gather_kernel(T *__restrict__ dst, const T *__restrict__ src,
+                                                     const uint32_t *__restrict__ perm, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) dst[i] = src[perm[i]];
+}
+
+inline dim3 grid_1d(uint64_t n) {
+    uint64_t b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    if (b == 0) b = 1;
+    return dim3((uint32_t)b);
+}
+
+int upload_coo(ptiDevCOO **out, const ptiSparseTensor *X, hipStream_t s) {
+    static const char *M = "DevCOO Upload";
+    PTI_CHECK(require_device(M), M);
+    const uint32_t N = X->nmodes;
+    if (N < 2 || N > 8) PTI_FAIL(PTIERR_VALUE_ERROR, M, "nmodes must be in [2, 8]");
+    if (X->nnz >= (1ull << 32)) PTI_FAIL(PTIERR_VALUE_ERROR, M, "nnz must be < 2^32");
+    // Coordinates index factor and output rows on the device: reject any that
+    // lie outside their mode.
+    for (uint32_t m = 0; m < N; ++m) {
+        std::atomic<bool> bad(false);
+        const uint32_t *d = X->inds[m].data;
+        const uint32_t lim = X->ndims[m];
+        pti::parallel_for(X->nnz, 0, [&](uint64_t b, uint64_t e, int) {
+            for (uint64_t z = b; z < e; ++z)
+                if (d[z] >= lim) {
+                    bad = true;
+                    return;
+                }
+        });
+        if (bad.load()) PTI_FAIL(PTIERR_VALUE_ERROR, M, "coordinate outside its mode");
+    }
+    ptiDevCOO *t = new ptiDevCOO();
+    t->nmodes = N;
+    t->nnz = X->nnz;
+    for (uint32_t m = 0; m < N; ++m) t->ndims[m] = X->ndims[m];
+    int rc = 0;
+    auto put = [&](Buffer &buf, const void *src, size_t bytes) -> int {
+        PTI_HIP(buf.alloc(bytes), M);
+        if (bytes) PTI_HIP(hipMemcpyAsync(buf.ptr, src, bytes, hipMemcpyHostToDevice, s), M);
+        return 0;
+    };
+    for (uint32_t m = 0; m < N && !rc; ++m) rc = put(t->inds[m], X->inds[m].data, X->nnz * 4);
+    if (!rc) rc = put(t->vals, X->values.data, X->nnz * 4);
+    hipError_t e = hipStreamSynchronize(s);
+    if (!rc && e != hipSuccess) {
+        pti_ComplainError(M, hip_code(e), __FILE__, __LINE__, hipGetErrorString(e));
+        rc = hip_code(e);
+    }
+    if (rc) {
+        delete t;
+        return rc;
+    }
+    *out = t;
+    return 0;
+}
+
+int prepare_mode(ptiDevCOO *t, ptiIndex mode, hipStream_t s) {
+    static const char *M = "DevCOO Prepare";
+    if (mode >= t->nmodes) PTI_FAIL(PTIERR_VALUE_ERROR, M, "mode out of range");
+    std::lock_guard<std::mutex> lk(t->mu);
+    if (t->ready[mode]) return 0;
+    const uint64_t n = t->nnz;
+    const uint32_t N = t->nmodes;
+    for (uint32_t m = 0; m < N; ++m) PTI_HIP(t->sinds[mode][m].alloc(n * 4), M);
+    PTI_HIP(t->svals[mode].alloc(n * 4), M);
+    if (n) {
+        Buffer keys_out, ids_in, ids_out, tmp;
+        PTI_HIP(keys_out.alloc(n * 4), M);
+        PTI_HIP(ids_in.alloc(n * 4), M);
+        PTI_HIP(ids_out.alloc(n * 4), M);
+        hipLaunchKernelGGL(iota_kernel, grid_1d(n), dim3(256), 0, s, ids_in.as<uint32_t>(), n);
+        PTI_HIP(hipGetLastError(), M);
+        unsigned end_bit = 1;
+        while (end_bit < 32 && (1ull << end_bit) < (uint64_t)t->ndims[mode]) ++end_bit;
+        size_t tmp_bytes = 0;
+        PTI_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, t->inds[mode].as<uint32_t>(), keys_out.as<uint32_t>(),
+                                          ids_in.as<uint32_t>(), ids_out.as<uint32_t>(), (size_t)n, 0u, end_bit, s),
+                M);
+        PTI_HIP(tmp.alloc(tmp_bytes), M);
+        PTI_HIP(rocprim::radix_sort_pairs(tmp.ptr, tmp_bytes, t->inds[mode].as<uint32_t>(), keys_out.as<uint32_t>(),
+                                          ids_in.as<uint32_t>(), ids_out.as<uint32_t>(), (size_t)n, 0u, end_bit, s),
+                M);
+        for (uint32_t m = 0; m < N; ++m) {
+            hipLaunchKernelGGL(gather_kernel<uint32_t>, grid_1d(n), dim3(256), 0, s, t->sinds[mode][m].as<uint32_t>(),
+                               t->inds[m].as<uint32_t>(), ids_out.as<uint32_t>(), n);
+            PTI_HIP(hipGetLastError(), M);
+        }
+        hipLaunchKernelGGL(gather_kernel<float>, grid_1d(n), dim3(256), 0, s, t->svals[mode].as<float>(),
+                           t->vals.as<float>(), ids_out.as<uint32_t>(), n);
+        PTI_HIP(hipGetLastError(), M);
+        // The temporaries are freed at scope exit: finish the work first.
+        PTI_HIP(hipStreamSynchronize(s), M);
+    }
+    t->ready[mode] = true;
+    return 0;
+}
+
+}  // namespace
+
+int run_coo(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, const ptiIndex mats_order[],
+            ptiIndex mode, hipStream_t s, const char *M) {
+    const uint32_t N = t->nmodes;
+    if (mode >= N) PTI_FAIL(PTIERR_VALUE_ERROR, M, "mode out of range");
+    if (R == 0 || stride < R) PTI_FAIL(PTIERR_SHAPE_MISMATCH, M, "bad rank or stride");
+    for (uint32_t i = 1; i < N; ++i)
+        if (mats_order[i] >= N) PTI_FAIL(PTIERR_VALUE_ERROR, M, "mats_order entry out of range");
+    PTI_CHECK(prepare_mode(t, mode, s), M);
+    float *out = dmats[N];
+    PTI_HIP(zero_floats(out, (uint64_t)t->ndims[mode] * stride, s), M);
+    if (t->nnz == 0) return 0;
+    COOArgs c;
+    for (uint32_t m = 0; m < N; ++m) c.inds[m] = t->sinds[mode][m].as<uint32_t>();
+    c.vals = t->svals[mode].as<float>();
+    c.nnz = t->nnz;
+    c.nchunks = (uint32_t)((t->nnz + CHUNK - 1) / CHUNK);
+    const ptiIndex tile = 256;
+    for (ptiIndex c0 = 0; c0 < R; c0 += tile) {
+        const ptiIndex Rt = std::min<ptiIndex>(tile, R - c0);
+        Shape sh;
+        if (!choose_shape(Rt, stride, stride, sh)) PTI_FAIL(PTIERR_VALUE_ERROR, M, "unsupported rank tile");
+        FactorArgs a;
+        for (uint32_t m = 0; m < N; ++m) a.fac[m] = dmats[m] + c0;
+        a.out = out + c0;
+        a.R = Rt;
+        a.stride = stride;
+        a.out_stride = stride;
+        a.mode = mode;
+        for (uint32_t i = 0; i < N; ++i) a.order[i] = mats_order[i];
+        PTI_HIP(launch_coo(N, c, a, sh, s), M);
+    }
+    return 0;
+}
+
+}  // namespace dev
+}  // namespace pti
+
+using namespace pti::dev;
+
+extern "C" {
+
+int ptiDevCOOUpload(ptiDevCOO **out, ptiSparseTensor const *X, void *stream) {
+    return upload_coo(out, X, static_cast<hipStream_t>(stream));
+}
+
+int ptiDevCOOPrepareMode(ptiDevCOO *t, ptiIndex mode, void *stream) {
+    return prepare_mode(t, mode, static_cast<hipStream_t>(stream));
+}
+
+void ptiDevCOOFree(ptiDevCOO *t) { delete t; }
+
+int ptiDevMTTKRPCOO(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, ptiIndex const mats_order[],
+                    ptiIndex mode, void *stream) {
+    return run_coo(t, dmats, R, stride, mats_order, mode, static_cast<hipStream_t>(stream), "DevCOO MTTKRP");
+}
+
+int ptiRegisterSparseTensor(ptiSparseTensor const *X) {
+    ptiDevCOO *t = nullptr;
+    PTI_CHECK(upload_coo(&t, X, nullptr), "Register SpTns");
+    std::lock_guard<std::mutex> lk(g_coo_mu);
+    auto it = g_coo_reg.find(X);
+    if (it != g_coo_reg.end()) delete it->second;
+    g_coo_reg[X] = t;
+    return 0;
+}
+
+int ptiUnregisterSparseTensor(ptiSparseTensor const *X) {
+    std::lock_guard<std::mutex> lk(g_coo_mu);
+    auto it = g_coo_reg.find(X);
+    if (it == g_coo_reg.end()) PTI_FAIL(PTIERR_VALUE_ERROR, "Unregister SpTns", "tensor is not registered");
+    delete it->second;
+    g_coo_reg.erase(it);
+    return 0;
+}
+
+int ptiMTTKRP(ptiSparseTensor const *const X, ptiMatrix *mats[], ptiIndex const mats_order[], ptiIndex const mode) {
+    static const char *M = "GPU  SpTns MTTKRP";
+    const ptiIndex N = X->nmodes;
+    PTI_CHECK(check_mttkrp_args(N, X->ndims, mats, mats_order, mode, M), M);
+    PTI_CHECK(require_device(M), M);
+    hipStream_t s = nullptr;
+    ptiDevCOO *t = nullptr;
+    ptiDevCOO *owned = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_coo_mu);
+        auto it = g_coo_reg.find(X);
+        if (it != g_coo_reg.end()) t = it->second;
+    }
+    if (!t) {
+        PTI_CHECK(upload_coo(&owned, X, s), M);
+        t = owned;
+    }
+    StagedFactors sf;
+    int rc = stage_factors(N, mats, mats_order, mode, sf, s, M);
+    if (!rc) rc = run_coo(t, sf.ptrs, mats[N]->ncols, mats[0]->stride, mats_order, mode, s, M);
+    if (!rc) rc = fetch_output(N, mats, mode, sf, s, M);
+    delete owned;
+    return rc;
+}
+
+int ptiCudaMTTKRP(ptiSparseTensor const *const X, ptiMatrix *mats[], ptiIndex *const mats_order, ptiIndex const mode,
+                  ptiIndex const impl_num) {
+    (void)impl_num;
+    return ptiMTTKRP(X, mats, mats_order, mode);
+}
+
+}  // extern "C"

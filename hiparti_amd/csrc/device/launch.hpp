@@ -1,0 +1,82 @@
+// Launch entry points of the MTTKRP kernel instantiations.  Each tensor order
+// N in [2, 8] is compiled in its own object (mttkrp_inst.hip built with
+// -DPTI_NMODES=N); this header declares the per-N launchers and a dispatcher.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mttkrp_kernels.hpp"
+
+namespace pti {
+namespace dev {
+
+// Lanes per nonzero G and ranks per lane V chosen for a rank R and strides.
+struct Shape {
+    int V;
+    int G;
+};
+
+// V = 4 when R and every stride are multiples of 4 (16-byte row pieces),
+// else 2, else 1; G = ceil(R / V) rounded up to a power of two in [4, 64].
+// Returns false when R exceeds 64 * V (callers split the rank into 256-column tiles).
+inline bool choose_shape(uint32_t R, uint32_t stride, uint32_t out_stride, Shape &sh) {
+    if (R == 0) return false;
+    int V = 1;
+    if (R % 4 == 0 && stride % 4 == 0 && out_stride % 4 == 0)
+        V = 4;
+    else if (R % 2 == 0 && stride % 2 == 0 && out_stride % 2 == 0)
+        V = 2;
+    uint32_t lanes = (R + V - 1) / V;
+    int G = 4;
+    while ((uint32_t)G < lanes) G <<= 1;
+    if (G > 64) return false;
+    sh.V = V;
+    sh.G = G;
+    return true;
+}
+
+#define PTI_DECLARE_LAUNCHERS(N)                                                                       \
+    hipError_t launch_hicoo_##N(const HiCOOArgs &h, const FactorArgs &a, Shape sh, bool bf16, hipStream_t s); \
+    hipError_t launch_coo_##N(const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s);        \
+    hipError_t launch_sort_chunks_##N(const HiCOOArgs &h, uint32_t mode, uint8_t *perm, hipStream_t s);
+
+PTI_DECLARE_LAUNCHERS(2)
+PTI_DECLARE_LAUNCHERS(3)
+PTI_DECLARE_LAUNCHERS(4)
+PTI_DECLARE_LAUNCHERS(5)
+PTI_DECLARE_LAUNCHERS(6)
+PTI_DECLARE_LAUNCHERS(7)
+PTI_DECLARE_LAUNCHERS(8)
+#undef PTI_DECLARE_LAUNCHERS
+
+#define PTI_DISPATCH_N(N, call)      \
+    switch (N) {                     \
+        case 2: return call(2);      \
+        case 3: return call(3);      \
+        case 4: return call(4);      \
+        case 5: return call(5);      \
+        case 6: return call(6);      \
+        case 7: return call(7);      \
+        case 8: return call(8);      \
+        default: return hipErrorInvalidValue; \
+    }
+
+inline hipError_t launch_hicoo(uint32_t N, const HiCOOArgs &h, const FactorArgs &a, Shape sh, bool bf16,
+                               hipStream_t s) {
+#define PTI_CALL(n) launch_hicoo_##n(h, a, sh, bf16, s)
+    PTI_DISPATCH_N(N, PTI_CALL)
+#undef PTI_CALL
+}
+inline hipError_t launch_coo(uint32_t N, const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s) {
+#define PTI_CALL(n) launch_coo_##n(c, a, sh, s)
+    PTI_DISPATCH_N(N, PTI_CALL)
+#undef PTI_CALL
+}
+inline hipError_t launch_sort_chunks(uint32_t N, const HiCOOArgs &h, uint32_t mode, uint8_t *perm, hipStream_t s) {
+#define PTI_CALL(n) launch_sort_chunks_##n(h, mode, perm, s)
+    PTI_DISPATCH_N(N, PTI_CALL)
+#undef PTI_CALL
+}
+
+}  // namespace dev
+}  // namespace pti

@@ -1,0 +1,94 @@
+// Kernel instantiations for one tensor order.  The build compiles this file
+// once per order with -DPTI_NMODES=N (N = 2..8) so the instantiations build in
+// parallel.
+#include "launch.hpp"
+
+#ifndef PTI_NMODES
+#error "compile with -DPTI_NMODES=<tensor order>"
+#endif
+
+#define PTI_CAT_(a, b) a##b
+#define PTI_CAT(a, b) PTI_CAT_(a, b)
+
+namespace pti {
+namespace dev {
+
+namespace {
+constexpr int N = PTI_NMODES;
+
+inline uint32_t grid_for_chunks(uint32_t nchunks) {
+    return (nchunks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+}
+
+template <int V, int G>
+hipError_t hicoo_vg(const HiCOOArgs &h, const FactorArgs &a, bool bf16, hipStream_t s) {
+    if (h.nchunks == 0) return hipSuccess;
+    dim3 grid(grid_for_chunks(h.nchunks)), block(256);
+    if (bf16)
+        hipLaunchKernelGGL((hicoo_mttkrp_kernel<N, V, G, uint16_t>), grid, block, 0, s, h, a);
+    else
+        hipLaunchKernelGGL((hicoo_mttkrp_kernel<N, V, G, float>), grid, block, 0, s, h, a);
+    return hipGetLastError();
+}
+
+template <int V, int G>
+hipError_t coo_vg(const COOArgs &c, const FactorArgs &a, hipStream_t s) {
+    if (c.nchunks == 0) return hipSuccess;
+    dim3 grid(grid_for_chunks(c.nchunks)), block(256);
+    hipLaunchKernelGGL((coo_mttkrp_kernel<N, V, G, float>), grid, block, 0, s, c, a);
+    return hipGetLastError();
+}
+
+template <int V>
+hipError_t hicoo_v(const HiCOOArgs &h, const FactorArgs &a, int G, bool bf16, hipStream_t s) {
+    switch (G) {
+        case 4: return hicoo_vg<V, 4>(h, a, bf16, s);
+        case 8: return hicoo_vg<V, 8>(h, a, bf16, s);
+        case 16: return hicoo_vg<V, 16>(h, a, bf16, s);
+        case 32: return hicoo_vg<V, 32>(h, a, bf16, s);
+        case 64: return hicoo_vg<V, 64>(h, a, bf16, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int V>
+hipError_t coo_v(const COOArgs &c, const FactorArgs &a, int G, hipStream_t s) {
+    switch (G) {
+        case 4: return coo_vg<V, 4>(c, a, s);
+        case 8: return coo_vg<V, 8>(c, a, s);
+        case 16: return coo_vg<V, 16>(c, a, s);
+        case 32: return coo_vg<V, 32>(c, a, s);
+        case 64: return coo_vg<V, 64>(c, a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+}  // namespace
+
+hipError_t PTI_CAT(launch_hicoo_, PTI_NMODES)(const HiCOOArgs &h, const FactorArgs &a, Shape sh, bool bf16,
+                                              hipStream_t s) {
+    switch (sh.V) {
+        case 4: return hicoo_v<4>(h, a, sh.G, bf16, s);
+        case 2: return hicoo_v<2>(h, a, sh.G, bf16, s);
+        case 1: return hicoo_v<1>(h, a, sh.G, bf16, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t PTI_CAT(launch_coo_, PTI_NMODES)(const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s) {
+    switch (sh.V) {
+        case 4: return coo_v<4>(c, a, sh.G, s);
+        case 2: return coo_v<2>(c, a, sh.G, s);
+        case 1: return coo_v<1>(c, a, sh.G, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t PTI_CAT(launch_sort_chunks_, PTI_NMODES)(const HiCOOArgs &h, uint32_t mode, uint8_t *perm,
+                                                    hipStream_t s) {
+    if (h.nchunks == 0) return hipSuccess;
+    hipLaunchKernelGGL((hicoo_sort_chunks_kernel<N>), dim3(h.nchunks), dim3(256), 0, s, h, mode, perm);
+    return hipGetLastError();
+}
+
+}  // namespace dev
+}  // namespace pti

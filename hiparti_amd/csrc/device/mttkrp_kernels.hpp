@@ -1,0 +1,352 @@
+// MTTKRP kernels for gfx950 (wave64).
+//
+// Work decomposition.  The nonzeros of the tensor (in storage order) are cut
+// into chunks of CHUNK = 256 consecutive nonzeros; one wavefront owns one
+// chunk, a 256-thread workgroup owns four.  Workgroup ids are remapped so
+// that consecutive chunks land on the same XCD (the hardware deals
+// workgroups to the 8 XCDs round-robin), which keeps the factor and output
+// rows of neighbouring chunks in one L2.
+//
+// 1. Staging.  The wave decodes its chunk into LDS: full coordinates of every
+//    nonzero in every mode and the values.  For HiCOO the block of nonzero z
+//    is found by binary search in the chunk's slice of bptr (held in LDS) and
+//    the coordinate is (binds << sb_bits) + eind; for COO the coordinates are
+//    read directly.
+// 2. Processing order.  The chunk is walked in the order of an optional
+//    per-mode permutation perm (1 byte per nonzero) that sorts the chunk by
+//    output row; the COO path uses per-mode copies already sorted by the
+//    output index, so it walks the chunk in storage order.
+// 3. Arithmetic.  The 64 lanes are split into NG = 64/G groups of G lanes; a
+//    group handles one nonzero per step and each lane V consecutive ranks
+//    (float4 / float2 / float loads of the factor rows, or 4/2/1 bf16 values
+//    for the BF16 form).  The contribution v * prod_k A_k[i_k][r] is formed in
+//    mats_order order, in FP32.
+// 4. Reduction.  Within a step the NG contributions are combined by a
+//    segmented inclusive scan over groups keyed on the output row; the last
+//    group of a segment adds the segment sum to the output with one FP32
+//    atomic per rank, except that a segment still open at the end of a step is
+//    carried into the next step.  Each output row therefore receives one
+//    atomic per (chunk, row) run instead of one per nonzero.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pti {
+namespace dev {
+
+constexpr int CHUNK = 256;          // nonzeros per wave chunk
+constexpr int WAVES_PER_BLOCK = 4;  // waves (chunks) per 256-thread workgroup
+constexpr uint32_t INVALID_ROW = 0xFFFFFFFFu;
+constexpr int MAX_MODES = 8;
+
+struct FactorArgs {
+    const void *fac[MAX_MODES];  // factor matrix base per mode (float* or bf16 bits)
+    float *out;                  // output, rows of the target mode
+    uint32_t R;                  // rank (columns used)
+    uint32_t stride;             // factor row stride (elements)
+    uint32_t out_stride;         // output row stride (elements)
+    uint32_t mode;               // target mode
+    uint32_t order[MAX_MODES];   // order[1..N-1]: factor modes in product order
+};
+
+struct HiCOOArgs {
+    const uint32_t *bptr;       // nb + 1 nonzero offsets (32-bit)
+    const uint32_t *binds;      // nb * N block coordinates, block-major
+    const uint32_t *einds;      // nnz * EW packed 8-bit element offsets
+    const float *vals;          // nnz values
+    const uint32_t *chunk_blk;  // nchunks + 1: block holding nonzero c*CHUNK; [nchunks] = nb
+    const uint8_t *perm;        // nullable: per-chunk row-sorted order for the target mode
+    uint64_t nnz;
+    uint32_t nb;
+    uint32_t nchunks;
+    uint32_t sb_bits;
+};
+
+struct COOArgs {
+    const uint32_t *inds[MAX_MODES];  // coordinates per mode (sorted by the target mode)
+    const float *vals;
+    uint64_t nnz;
+    uint32_t nchunks;
+};
+
+__device__ __forceinline__ uint32_t remap_block(uint32_t bid, uint32_t nblocks) {
+    // Hardware assigns workgroup bid to XCD bid % 8.  Give XCD x a contiguous
+    // range of logical workgroups so neighbouring chunks share an L2.
+    const uint32_t nx = 8;
+    uint32_t per = nblocks / nx, rem = nblocks % nx;
+    uint32_t x = bid % nx, k = bid / nx;
+    // XCD x holds logical ids [start_x, start_x + per + (x < rem)).
+    uint32_t start = x * per + (x < rem ? x : rem);
+    return start + k;
+}
+
+template <typename FT, int V>
+struct FactorLoad;
+
+template <>
+struct FactorLoad<float, 4> {
+    __device__ __forceinline__ static void load(const void *base, uint64_t off, float *o) {
+        float4 t = *reinterpret_cast<const float4 *>(static_cast<const float *>(base) + off);
+        o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+    }
+};
+template <>
+struct FactorLoad<float, 2> {
+    __device__ __forceinline__ static void load(const void *base, uint64_t off, float *o) {
+        float2 t = *reinterpret_cast<const float2 *>(static_cast<const float *>(base) + off);
+        o[0] = t.x; o[1] = t.y;
+    }
+};
+template <>
+struct FactorLoad<float, 1> {
+    __device__ __forceinline__ static void load(const void *base, uint64_t off, float *o) {
+        o[0] = static_cast<const float *>(base)[off];
+    }
+};
+__device__ __forceinline__ float bf16_bits_to_float(uint32_t b) { return __uint_as_float(b << 16); }
+template <>
+struct FactorLoad<uint16_t, 4> {
+    __device__ __forceinline__ static void load(const void *base, uint64_t off, float *o) {
+        uint2 t = *reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(base) + off);
+        o[0] = bf16_bits_to_float(t.x & 0xFFFFu); o[1] = bf16_bits_to_float(t.x >> 16);
+        o[2] = bf16_bits_to_float(t.y & 0xFFFFu); o[3] = bf16_bits_to_float(t.y >> 16);
+    }
+};
+template <>
+struct FactorLoad<uint16_t, 2> {
+    __device__ __forceinline__ static void load(const void *base, uint64_t off, float *o) {
+        uint32_t t = *reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(base) + off);
+        o[0] = bf16_bits_to_float(t & 0xFFFFu); o[1] = bf16_bits_to_float(t >> 16);
+    }
+};
+template <>
+struct FactorLoad<uint16_t, 1> {
+    __device__ __forceinline__ static void load(const void *base, uint64_t off, float *o) {
+        o[0] = bf16_bits_to_float(static_cast<const uint16_t *>(base)[off]);
+    }
+};
+
+template <int N>
+struct Staging {
+    uint32_t coord[N][CHUNK];
+    float val[CHUNK];
+    uint32_t bptr[CHUNK + 4];
+};
+
+// Decode chunk `chunk` of a HiCOO tensor into st (one wave, 64 lanes).
+// Returns the number of nonzeros in the chunk.
+template <int N>
+__device__ __forceinline__ uint32_t stage_hicoo(const HiCOOArgs &h, uint32_t chunk, Staging<N> &st, int lane) {
+    constexpr int EW = (N + 3) / 4;
+    const uint64_t z0 = (uint64_t)chunk * CHUNK;
+    const uint64_t rem = h.nnz - z0;
+    const uint32_t cnt = rem < CHUNK ? (uint32_t)rem : (uint32_t)CHUNK;
+    const uint32_t b0 = h.chunk_blk[chunk];
+    const uint32_t b1 = h.chunk_blk[chunk + 1];
+    const uint32_t blast = (b1 + 1 < h.nb) ? b1 + 1 : h.nb;   // inclusive bptr index
+    const uint32_t nbp = blast - b0 + 1;                      // <= CHUNK + 2
+    for (uint32_t j = lane; j < nbp; j += 64) st.bptr[j] = h.bptr[b0 + j];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t zl = lane; zl < cnt; zl += 64) {
+        const uint32_t z = (uint32_t)(z0 + zl);
+        // Largest j with bptr[b0 + j] <= z, j in [0, nbp - 2].
+        uint32_t lo = 0, hi = nbp - 2;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (st.bptr[mid] <= z) lo = mid; else hi = mid - 1;
+        }
+        const uint32_t b = b0 + lo;
+        uint32_t ew[EW];
+#pragma unroll
+        for (int w = 0; w < EW; ++w) ew[w] = h.einds[(uint64_t)z * EW + w];
+#pragma unroll
+        for (int m = 0; m < N; ++m) {
+            uint32_t e = (ew[m >> 2] >> ((m & 3) * 8)) & 0xFFu;
+            st.coord[m][zl] = (h.binds[(uint64_t)b * N + m] << h.sb_bits) + e;
+        }
+        st.val[zl] = h.vals[z];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return cnt;
+}
+
+template <int N>
+__device__ __forceinline__ uint32_t stage_coo(const COOArgs &c, uint32_t chunk, Staging<N> &st, int lane) {
+    const uint64_t z0 = (uint64_t)chunk * CHUNK;
+    const uint64_t rem = c.nnz - z0;
+    const uint32_t cnt = rem < CHUNK ? (uint32_t)rem : (uint32_t)CHUNK;
+    for (uint32_t zl = lane; zl < cnt; zl += 64) {
+#pragma unroll
+        for (int m = 0; m < N; ++m) st.coord[m][zl] = c.inds[m][z0 + zl];
+        st.val[zl] = c.vals[z0 + zl];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return cnt;
+}
+
+// Contribution of sorted position p of the staged chunk for this lane's V ranks.
+template <int N, int V, typename FT>
+__device__ __forceinline__ uint32_t contribution(const FactorArgs &a, const Staging<N> &st, const uint8_t *perm,
+                                                 uint32_t p, uint32_t cnt, uint32_t col, float *out) {
+#pragma unroll
+    for (int k = 0; k < V; ++k) out[k] = 0.f;
+    if (p >= cnt) return INVALID_ROW;
+    const uint32_t li = perm ? (uint32_t)perm[p] : p;
+    const uint32_t row = st.coord[a.mode][li];
+    if (col < a.R) {
+        const float v = st.val[li];
+        float t[V];
+        FactorLoad<FT, V>::load(a.fac[a.order[1]], (uint64_t)st.coord[a.order[1]][li] * a.stride + col, t);
+#pragma unroll
+        for (int k = 0; k < V; ++k) out[k] = v * t[k];
+#pragma unroll
+        for (int i = 2; i < N; ++i) {
+            FactorLoad<FT, V>::load(a.fac[a.order[i]], (uint64_t)st.coord[a.order[i]][li] * a.stride + col, t);
+#pragma unroll
+            for (int k = 0; k < V; ++k) out[k] *= t[k];
+        }
+    }
+    return row;
+}
+
+template <int V>
+__device__ __forceinline__ void flush(const FactorArgs &a, uint32_t row, uint32_t col, const float *val) {
+    if (col < a.R) {
+        float *dst = a.out + (uint64_t)row * a.out_stride + col;
+#pragma unroll
+        for (int k = 0; k < V; ++k) unsafeAtomicAdd(dst + k, val[k]);
+    }
+}
+
+// Walk a staged chunk in sorted order, reduce runs of equal output rows and
+// add them to the output (see the header comment, steps 3-4).
+template <int N, int V, int G, typename FT>
+__device__ __forceinline__ void process_chunk(const FactorArgs &a, const Staging<N> &st, const uint8_t *perm,
+                                              uint32_t cnt, int lane) {
+    constexpr int NG = 64 / G;
+    constexpr int STEPS = CHUNK / NG;
+    const int g = lane / G;
+    const int s = lane % G;
+    const uint32_t col = (uint32_t)s * V;
+
+    float carry[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) carry[k] = 0.f;
+    uint32_t carry_row = INVALID_ROW;
+
+    float nxt[V];
+    uint32_t nxt_row = contribution<N, V, FT>(a, st, perm, (uint32_t)g, cnt, col, nxt);
+    const int nsteps = (int)((cnt + NG - 1) / NG);
+    for (int step = 0; step < nsteps; ++step) {
+        float val[V];
+#pragma unroll
+        for (int k = 0; k < V; ++k) val[k] = nxt[k];
+        uint32_t row = nxt_row;
+        if (step + 1 < nsteps)
+            nxt_row = contribution<N, V, FT>(a, st, perm, (uint32_t)((step + 1) * NG + g), cnt, col, nxt);
+
+        // Group 0 continues or closes the segment carried from the last step.
+        if (g == 0 && carry_row != INVALID_ROW) {
+            if (row == carry_row) {
+#pragma unroll
+                for (int k = 0; k < V; ++k) val[k] += carry[k];
+            } else {
+                flush<V>(a, carry_row, col, carry);
+            }
+        }
+        // Segment heads: first group, or row differs from the previous group.
+        uint32_t prev_row = __shfl_up(row, G, 64);
+        int f = (g == 0) || (prev_row != row);
+#pragma unroll
+        for (int d = 1; d < NG; d <<= 1) {
+            int of = __shfl_up(f, d * G, 64);
+            float ov[V];
+#pragma unroll
+            for (int k = 0; k < V; ++k) ov[k] = __shfl_up(val[k], d * G, 64);
+            if (g >= d) {
+                if (!f) {
+#pragma unroll
+                    for (int k = 0; k < V; ++k) val[k] += ov[k];
+                }
+                f |= of;
+            }
+        }
+        uint32_t next_row = __shfl_down(row, G, 64);
+        const bool last_group = (g == NG - 1);
+        if (!last_group && next_row != row && row != INVALID_ROW) flush<V>(a, row, col, val);
+        // The last group's segment stays open: hand it to group 0.
+        const int src = (NG - 1) * G + s;
+        carry_row = __shfl(row, (NG - 1) * G, 64);
+#pragma unroll
+        for (int k = 0; k < V; ++k) carry[k] = __shfl(val[k], src, 64);
+    }
+    if (g == 0 && carry_row != INVALID_ROW) flush<V>(a, carry_row, col, carry);
+}
+
+template <int N, int V, int G, typename FT>
+__global__ void __launch_bounds__(256) hicoo_mttkrp_kernel(HiCOOArgs h, FactorArgs a) {
+    __shared__ Staging<N> st_all[WAVES_PER_BLOCK];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t chunk = remap_block(blockIdx.x, gridDim.x) * WAVES_PER_BLOCK + wid;
+    if (chunk >= h.nchunks) return;
+    Staging<N> &st = st_all[wid];
+    const uint32_t cnt = stage_hicoo<N>(h, chunk, st, lane);
+    const uint8_t *perm = h.perm ? h.perm + (uint64_t)chunk * CHUNK : nullptr;
+    process_chunk<N, V, G, FT>(a, st, perm, cnt, lane);
+}
+
+template <int N, int V, int G, typename FT>
+__global__ void __launch_bounds__(256) coo_mttkrp_kernel(COOArgs c, FactorArgs a) {
+    __shared__ Staging<N> st_all[WAVES_PER_BLOCK];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t chunk = remap_block(blockIdx.x, gridDim.x) * WAVES_PER_BLOCK + wid;
+    if (chunk >= c.nchunks) return;
+    Staging<N> &st = st_all[wid];
+    const uint32_t cnt = stage_coo<N>(c, chunk, st, lane);
+    process_chunk<N, V, G, FT>(a, st, nullptr, cnt, lane);
+}
+
+// Builds perm for one mode: each 256-thread workgroup sorts one chunk's
+// (row, local index) keys with a bitonic network in LDS.
+template <int N>
+__global__ void __launch_bounds__(256) hicoo_sort_chunks_kernel(HiCOOArgs h, uint32_t mode, uint8_t *perm) {
+    __shared__ Staging<N> st;
+    __shared__ uint64_t key[CHUNK];
+    const uint32_t chunk = blockIdx.x;
+    const int t = threadIdx.x;
+    uint32_t cnt = 0;
+    if (t < 64) cnt = stage_hicoo<N>(h, chunk, st, t);
+    __syncthreads();
+    {
+        const uint64_t z0 = (uint64_t)chunk * CHUNK;
+        const uint64_t rem = h.nnz - z0;
+        cnt = rem < CHUNK ? (uint32_t)rem : (uint32_t)CHUNK;
+    }
+    key[t] = (uint32_t)t < cnt ? (((uint64_t)st.coord[mode][t] << 8) | (uint64_t)t) : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= CHUNK; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            int ixj = t ^ j;
+            if (ixj > t) {
+                uint64_t x = key[t], y = key[ixj];
+                bool up = (t & k) == 0;
+                if ((x > y) == up) {
+                    key[t] = y;
+                    key[ixj] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if ((uint32_t)t < cnt) perm[(uint64_t)chunk * CHUNK + t] = (uint8_t)(key[t] & 0xFFu);
+}
+
+}  // namespace dev
+}  // namespace pti

@@ -1,0 +1,238 @@
+"""Writes the golden fixtures under tests/golden/ from the reference itself.
+
+Runs in the build container only: it loads oracle/_ref/libhiparti_ref.so,
+which oracle/ref.mk compiles from the reference's C sources under
+/root/reference, and records for each input
+  - the reference's COO -> HiCOO conversion (ptiSparseTensorToHiCOO, tk = 1):
+    every array of the HiCOO struct and the COO order it leaves behind;
+  - the reference's sequential ptiMTTKRP for every mode with three factor
+    sets: all ones (the benchmark CLI's choice, benchmark/tensor/mttkrp.c:159),
+    the reference initialisation i + j + 1 (src/matrix/matrix.c:67-74) and
+    seeded U(0,1) values;
+  - the reference's sequential ptiMTTKRPHiCOO for every mode (U(0,1) factors).
+Inputs: the reference's data files (data/tensors/*.tns, except the
+headerless 4d_3_16.tns that its loader cannot read) and seeded synthetic
+tensors drawn with libHiParTI's generator, including a 5-D tensor and a 3-D
+tensor with duplicate coordinates (both exercise the quicksort's order of
+equal keys) and a 3-D tensor with indices >= 2^24 (the Morton key ignores
+bits 24-31).  Inputs are stored in the fixtures, so the tests do not depend on
+the generator.
+
+    make -f oracle/ref.mk && python tests/golden/make_goldens.py
+"""
+import ctypes as C
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from hiparti_amd import _ffi as F  # noqa: E402
+from hiparti_amd.tensor import COOTensor, hicoo_struct_arrays  # noqa: E402
+
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libhiparti_ref.so")
+REF_DATA = "/root/reference/data/tensors"
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+ref = F.declare(C.CDLL(REF_LIB))
+ref.ptiMTTKRP.argtypes = [C.POINTER(F.ptiSparseTensor), C.POINTER(C.POINTER(F.ptiMatrix)),
+                          C.POINTER(F.ptiIndex), F.ptiIndex]
+
+
+def ref_tensor(ndims, inds, vals):
+    t = F.ptiSparseTensor()
+    inds = np.ascontiguousarray(inds, np.uint32)  # kept alive across the copies below
+    vals = np.ascontiguousarray(vals, np.float32)
+    N, nnz = inds.shape
+    assert ref.ptiNewSparseTensor(C.byref(t), N, (F.ptiIndex * N)(*ndims)) == 0
+    for m in range(N):
+        assert ref.ptiResizeIndexVector(C.byref(t.inds[m]), nnz) == 0
+        if nnz:
+            C.memmove(t.inds[m].data, inds[m].ctypes.data, nnz * 4)
+    assert ref.ptiResizeValueVector(C.byref(t.values), nnz) == 0
+    if nnz:
+        C.memmove(t.values.data, vals.ctypes.data, nnz * 4)
+    t.nnz = nnz
+    return t
+
+
+def coo_arrays(t):
+    N, nnz = int(t.nmodes), int(t.nnz)
+    inds = np.stack([np.ctypeslib.as_array(t.inds[m].data, shape=(nnz,)).copy() for m in range(N)])
+    vals = np.ctypeslib.as_array(t.values.data, shape=(nnz,)).copy()
+    return inds, vals
+
+
+def ref_matrix(a, nrows=None):
+    nrows = a.shape[0] if nrows is None else nrows
+    m = F.ptiMatrix()
+    assert ref.ptiNewMatrix(C.byref(m), nrows, a.shape[1]) == 0
+    full = np.ctypeslib.as_array(m.values, shape=(int(m.cap) * int(m.stride),)).reshape(int(m.cap), int(m.stride))
+    full[:] = 0
+    full[:a.shape[0], :a.shape[1]] = a
+    return m, full
+
+
+def ref_mttkrp(t, facs, mode, hicoo=None):
+    N = len(facs)
+    R = facs[0].shape[1]
+    mats = [ref_matrix(f) for f in facs]
+    out, outv = ref_matrix(np.zeros((1, R), np.float32), nrows=max(f.shape[0] for f in facs))
+    arr = (C.POINTER(F.ptiMatrix) * (N + 1))(*[C.pointer(m) for m, _ in mats], C.pointer(out))
+    order = (F.ptiIndex * N)(*([mode] + [(mode + i) % N for i in range(1, N)]))
+    if hicoo is None:
+        rc = ref.ptiMTTKRP(C.byref(t), arr, order, mode)
+    else:
+        rc = ref.ptiMTTKRPHiCOO(C.byref(hicoo), arr, order, mode)
+    assert rc == 0
+    res = outv[:facs[mode].shape[0], :R].copy()
+    for m, _ in mats:
+        ref.ptiFreeMatrix(C.byref(m))
+    ref.ptiFreeMatrix(C.byref(out))
+    return res
+
+
+def factor_sets(ndims, R, seed):
+    rng_facs = []
+    for m, d in enumerate(ndims):
+        # U(0,1) from the library's counter-based generator, via numpy-free path
+        from hiparti_amd.tensor import DenseMatrix
+        dm = DenseMatrix(d, R)
+        dm.uniform(seed + m)
+        rng_facs.append(dm.numpy())
+    ones = [np.ones((d, R), np.float32) for d in ndims]
+    ipj = [(np.arange(d)[:, None] + np.arange(R)[None, :] + 1).astype(np.float32) for d in ndims]
+    return {"const": ones, "ipj": ipj, "unif": rng_facs}
+
+
+def make_case(name, ndims, inds, vals, params, R, seed=7, ipj=True, mttkrp=True):
+    ndims = [int(x) for x in ndims]
+    N = len(ndims)
+    rec = {"ndims": np.array(ndims, np.uint32), "in_inds": inds.astype(np.uint32), "in_vals": vals.astype(np.float32),
+           "params": np.array(params, np.uint32), "R": np.uint32(R)}
+    sets = factor_sets(ndims, R, seed) if mttkrp else {}
+    if mttkrp and not ipj:
+        del sets["ipj"]
+    for m in range(N if mttkrp else 0):
+        rec["fac%d" % m] = sets["unif"][m]
+    # COO MTTKRP on the input order.
+    t = ref_tensor(ndims, inds, vals)
+    for kind, facs in sets.items():
+        for mode in range(N):
+            rec["coo_%s_m%d" % (kind, mode)] = ref_mttkrp(t, facs, mode)
+    ref.ptiFreeSparseTensor(C.byref(t))
+    # Conversion for each parameter set, then HiCOO MTTKRP on the first one.
+    for p, (sb, sk, sc) in enumerate(params):
+        t = ref_tensor(ndims, inds, vals)
+        h = F.ptiSparseTensorHiCOO()
+        mx = F.ptiNnzIndex()
+        assert ref.ptiSparseTensorToHiCOO(C.byref(h), C.byref(mx), C.byref(t), sb, sk, sc, 1) == 0
+        for k, v in hicoo_struct_arrays(h).items():
+            rec["p%d_%s" % (p, k)] = v
+        rec["p%d_max_nnzb" % p] = np.uint64(mx.value)
+        si, sv = coo_arrays(t)
+        rec["p%d_sorted_inds" % p] = si
+        rec["p%d_sorted_vals" % p] = sv
+        if p == 0 and mttkrp:
+            for mode in range(N):
+                rec["hicoo_unif_m%d" % mode] = ref_mttkrp(None, sets["unif"], mode, hicoo=h)
+        ref.ptiFreeSparseTensorHiCOO(C.byref(h))
+        ref.ptiFreeSparseTensor(C.byref(t))
+    path = os.path.join(OUT, name + ".npz")
+    np.savez_compressed(path, **rec)
+    print("wrote", path, "nnz", inds.shape[1], flush=True)
+
+
+def load_tns(fname):
+    t = F.ptiSparseTensor()
+    assert ref.ptiLoadSparseTensor(C.byref(t), 1, os.path.join(REF_DATA, fname).encode()) == 0
+    ndims = [int(t.ndims[m]) for m in range(int(t.nmodes))]
+    inds, vals = coo_arrays(t)
+    ref.ptiFreeSparseTensor(C.byref(t))
+    return ndims, inds, vals
+
+
+def synth(ndims, nnz, dist, zipf_s, permute, seed, lognormal=False):
+    t = COOTensor.synthetic(ndims, nnz, dist=dist, zipf_s=zipf_s, permute=permute, lognormal=lognormal, seed=seed)
+    return t.inds(), t.values()
+
+
+def digest(arrs):
+    h = hashlib.sha256()
+    for a in arrs:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def main():
+    P_SMALL = [(1, 2, 2), (2, 3, 4)]
+    for fname in ("3d_7.tns", "3d-24.tns", "3d_dense.tns"):
+        nd, i, v = load_tns(fname)
+        make_case("tns_" + fname.replace(".tns", "").replace("-", "_"), nd, i, v, P_SMALL, R=4)
+    nd, i, v = load_tns("3D_12031.tns")
+    make_case("tns_3D_12031", nd, i, v, [(2, 4, 4), (3, 5, 7), (7, 7, 7)], R=16)
+
+    rng = np.random.default_rng(1234)
+    # 3-D Zipf, relabelled modes.
+    nd = [1500, 1200, 400]
+    i, v = synth(nd, 8000, [1, 1, 1], [1.1] * 3, [1, 1, 1], seed=11)
+    make_case("syn3_zipf", nd, i, v, [(2, 4, 4), (7, 9, 9)], R=8, ipj=False)
+    # 3-D with indices >= 2^24 in mode 0 (Morton key drops bits 24-31).
+    nd = [(1 << 25) + 100, 300, 64]
+    i, v = synth(nd, 3000, [0, 0, 0], None, None, seed=12)
+    make_case("syn3_highbits", nd, i, v, [(2, 25, 25), (3, 26, 26)], R=8, ipj=False, mttkrp=False)
+    # 3-D with duplicate coordinates, shuffled order.
+    nd = [60, 50, 40]
+    i, v = synth(nd, 2000, [0, 0, 0], None, None, seed=13)
+    dup = rng.integers(0, i.shape[1], 600)
+    i = np.concatenate([i, i[:, dup]], axis=1)
+    v = np.concatenate([v, rng.random(600).astype(np.float32)])
+    perm = rng.permutation(i.shape[1])
+    make_case("syn3_dup", nd, i[:, perm], v[perm], [(2, 3, 3), (1, 4, 5)], R=8, ipj=False)
+    # 4-D, Zipf on modes 0-1.
+    nd = [400, 300, 50, 20]
+    i, v = synth(nd, 6000, [1, 1, 0, 0], [1.2, 1.2, 1, 1], [0, 0, 0, 0], seed=14, lognormal=True)
+    make_case("syn4", nd, i, v, [(2, 4, 4), (3, 6, 6)], R=12, ipj=False)
+    # 5-D: no Morton sort; the row-block quicksort's tie order is the result.
+    nd = [100, 80, 60, 30, 12]
+    i, v = synth(nd, 5000, [1, 1, 1, 0, 0], [1.1, 1.1, 1.1, 1, 1], [0, 0, 0, 0, 0], seed=15)
+    make_case("syn5", nd, i, v, [(2, 3, 3), (1, 2, 4)], R=8, ipj=False)
+    # 5-D in shuffled order.
+    perm = rng.permutation(i.shape[1])
+    make_case("syn5_shuffled", nd, i[:, perm], v[perm], [(2, 3, 3)], R=8, ipj=False)
+    # 2-D.
+    nd = [500, 400]
+    i, v = synth(nd, 3000, [1, 0], [1.1, 1], [1, 0], seed=16)
+    make_case("syn2", nd, i, v, [(2, 4, 4)], R=5, ipj=False)
+
+    # Conversion digests at larger scale (arrays too large to commit).
+    rec = {}
+    for name, nd, nnz, dist, s, permute, params in (
+            ("c0", [1000, 1000, 1000], 1000000, [0, 0, 0], None, [0, 0, 0], (7, 14, 14)),
+            ("c1s", [1 << 20, 1 << 20, 1 << 16], 2000000, [1, 1, 1], [1.1] * 3, [1, 1, 1], (7, 14, 14)),
+            ("c4s", [1000000, 500000, 100000, 1000, 24], 300000, [1, 1, 1, 0, 0], [1.1, 1.1, 1.1, 1, 1],
+             [0, 0, 0, 0, 0], (7, 14, 14))):
+        t = COOTensor.synthetic(nd, nnz, dist=dist, zipf_s=s, permute=permute, seed=42)
+        i, v = t.inds(), t.values()
+        rt = ref_tensor(nd, i, v)
+        h = F.ptiSparseTensorHiCOO()
+        mx = F.ptiNnzIndex()
+        assert ref.ptiSparseTensorToHiCOO(C.byref(h), C.byref(mx), C.byref(rt), *params, 1) == 0
+        a = hicoo_struct_arrays(h)
+        rec[name + "_input"] = digest([i, v])
+        rec[name + "_blocks"] = digest([a["bptr"], a["binds"], a["einds"], a["values"]])
+        rec[name + "_sched"] = digest([a["kptr"], a["cptr"], a["knnzs"], a["nkiters"]] +
+                                      [a["kschr_dat%d" % m] for m in range(len(nd))])
+        rec[name + "_nb"] = str(len(a["bptr"]) - 1)
+        ref.ptiFreeSparseTensorHiCOO(C.byref(h))
+        ref.ptiFreeSparseTensor(C.byref(rt))
+        print("digest", name, rec[name + "_nb"], flush=True)
+    with open(os.path.join(OUT, "digests.txt"), "w") as f:
+        for k in sorted(rec):
+            f.write("%s %s\n" % (k, rec[k]))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,214 @@
+"""GPU parity of the MTTKRP path (HIP kernels through the C ABI).
+
+Every result is compared with the FP64 oracle (relative Frobenius error
+<= 1e-5, the FP32 tolerance) and, where the reference's own FP32 output is in
+the fixtures, with that output (<= 1e-5).  Covers the drop-in entry points
+ptiMTTKRP / ptiMTTKRPHiCOO on the reference fixtures, orders 2-8, ranks
+1-300 (including ranks that are not multiples of 4 and ranks above one
+256-rank column tile), non-default mats_order, empty and single-nonzero
+tensors, registered (device-resident) tensors, the device API on torch
+tensors, and the BF16-factor form against the FP64 oracle evaluated on the
+same bf16-rounded factors.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden_cases, load_golden, rel_fro
+from hiparti_amd import COOTensor, DenseMatrix
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu(gpu_available):
+    if not gpu_available:
+        pytest.fail("no GPU visible: the GPU tests must run on the GPU box")
+
+
+def dense_mats(facs, out_rows, fill=np.nan):
+    mats = [DenseMatrix.from_numpy(f) for f in facs]
+    out = DenseMatrix(out_rows, facs[0].shape[1])
+    out.full_view()[:] = fill
+    return mats + [out]
+
+
+def run_coo(ndims, inds, vals, facs, mode, order=None):
+    t = COOTensor.from_arrays(ndims, inds, vals)
+    return t.MTTKRP(mode, dense_mats(facs, max(ndims)), order)
+
+
+def run_hicoo(ndims, inds, vals, facs, mode, sb=7, sk=7, sc=7, order=None):
+    t = COOTensor.from_arrays(ndims, inds, vals)
+    h = t.toHiCOO(sb, sk, sc)
+    return h.MTTKRP(mode, dense_mats(facs, max(ndims)), order)
+
+
+MTTKRP_CASES = [c for c in golden_cases() if "coo_unif_m0" in load_golden(c)]
+
+
+@pytest.mark.parametrize("case", MTTKRP_CASES)
+def test_dropin_coo_matches_reference_and_fp64(case, oracle):
+    g = load_golden(case)
+    N = len(g["ndims"])
+    R = int(g["R"])
+    sets = {"unif": [g["fac%d" % m] for m in range(N)],
+            "const": [np.ones((int(d), R), np.float32) for d in g["ndims"]]}
+    for kind, facs in sets.items():
+        for mode in range(N):
+            out = run_coo(g["ndims"], g["in_inds"], g["in_vals"], facs, mode)
+            f64 = oracle.mttkrp(g["in_inds"], g["in_vals"], facs, mode, kind="f64")
+            assert rel_fro(out, f64) <= TOL, (case, kind, mode)
+            assert rel_fro(out, g["coo_%s_m%d" % (kind, mode)]) <= TOL, (case, kind, mode)
+
+
+@pytest.mark.parametrize("case", MTTKRP_CASES)
+def test_dropin_hicoo_matches_reference_and_fp64(case, oracle):
+    g = load_golden(case)
+    N = len(g["ndims"])
+    sb, sk, sc = (int(x) for x in g["params"][0])
+    facs = [g["fac%d" % m] for m in range(N)]
+    t = COOTensor.from_arrays(g["ndims"], g["in_inds"], g["in_vals"])
+    h = t.toHiCOO(sb, sk, sc)
+    for mode in range(N):
+        out = h.MTTKRP(mode, dense_mats(facs, max(g["ndims"])))
+        f64 = oracle.mttkrp(g["in_inds"], g["in_vals"], facs, mode, kind="f64")
+        assert rel_fro(out, f64) <= TOL, (case, mode)
+        assert rel_fro(out, g["hicoo_unif_m%d" % mode]) <= TOL, (case, mode)
+
+
+SHAPES = [
+    # (ndims, nnz, R, sb, sk)
+    ([50, 40], 700, 8, 2, 3),
+    ([300, 200, 100], 6000, 1, 3, 5),
+    ([300, 200, 100], 6000, 3, 3, 5),
+    ([300, 200, 100], 6000, 5, 7, 7),
+    ([300, 200, 100], 6000, 32, 2, 4),
+    ([300, 200, 100], 6000, 33, 7, 9),
+    ([300, 200, 100], 6000, 64, 4, 8),
+    ([120, 90, 70], 3000, 100, 3, 5),
+    ([120, 90, 70], 3000, 300, 3, 5),
+    ([40, 30, 20, 10], 4000, 16, 2, 3),
+    ([30, 20, 15, 10, 8], 5000, 16, 1, 3),
+    ([12, 11, 10, 9, 8, 7], 4000, 8, 1, 2),
+    ([9, 8, 7, 6, 5, 4, 3], 3000, 12, 1, 1),
+    ([6, 6, 5, 5, 4, 4, 3, 3], 3000, 8, 1, 1),
+]
+
+
+@pytest.mark.parametrize("ndims,nnz,R,sb,sk", SHAPES)
+def test_random_shapes_against_fp64(ndims, nnz, R, sb, sk, oracle):
+    rng = np.random.default_rng(sum(ndims) + R)
+    t = COOTensor.synthetic(ndims, nnz, dist=[1] * len(ndims), zipf_s=[1.05] * len(ndims), seed=R)
+    inds, vals = t.inds(), t.values()
+    N = len(ndims)
+    facs = [rng.random((d, R)).astype(np.float32) for d in ndims]
+    for mode in range(N):
+        f64 = oracle.mttkrp(inds, vals, facs, mode, kind="f64")
+        assert rel_fro(run_coo(ndims, inds, vals, facs, mode), f64) <= TOL, ("coo", mode)
+        assert rel_fro(run_hicoo(ndims, inds, vals, facs, mode, sb, sk, sk), f64) <= TOL, ("hicoo", mode)
+    # A different product order changes rounding only.
+    order = [N - 1] + list(range(N - 1))[::-1]
+    f64 = oracle.mttkrp(inds, vals, facs, N - 1, order=order, kind="f64")
+    assert rel_fro(run_hicoo(ndims, inds, vals, facs, N - 1, sb, sk, sk, order=order), f64) <= TOL
+
+
+def test_empty_and_single_nonzero(oracle):
+    ndims = [7, 6, 5]
+    facs = [np.random.default_rng(0).random((d, 8)).astype(np.float32) for d in ndims]
+    empty = (np.zeros((3, 0), np.uint32), np.zeros(0, np.float32))
+    for mode in range(3):
+        assert np.all(run_coo(ndims, *empty, facs, mode) == 0)
+        assert np.all(run_hicoo(ndims, *empty, facs, mode, 1, 1, 1) == 0)
+    one = (np.array([[3], [2], [4]], np.uint32), np.array([2.5], np.float32))
+    for mode in range(3):
+        f64 = oracle.mttkrp(*one, facs, mode, kind="f64")
+        assert rel_fro(run_coo(ndims, *one, facs, mode), f64) <= 1e-7
+        assert rel_fro(run_hicoo(ndims, *one, facs, mode, 1, 1, 1), f64) <= 1e-7
+
+
+def test_output_rows_are_zeroed_and_padding_untouched():
+    ndims = [20, 10, 10]
+    t = COOTensor.from_arrays(ndims, np.array([[1], [2], [3]], np.uint32), np.ones(1, np.float32))
+    facs = [np.ones((d, 5), np.float32) for d in ndims]
+    mats = dense_mats(facs, 20, fill=7.0)
+    t.MTTKRP(0, mats)
+    full = mats[3].full_view()
+    expect = np.zeros((20, mats[3].stride()), np.float32)
+    expect[1, :5] = 1.0
+    np.testing.assert_array_equal(full[:20], expect)
+
+
+def test_registered_tensors_give_the_same_result(oracle):
+    from hiparti_amd import lib
+    import ctypes as C
+    g = load_golden("tns_3D_12031")
+    facs = [g["fac%d" % m] for m in range(3)]
+    t = COOTensor.from_arrays(g["ndims"], g["in_inds"], g["in_vals"])
+    h = t.toHiCOO(2, 4, 4)
+    assert lib().ptiRegisterHiCOO(C.byref(h.address)) == 0
+    assert lib().ptiRegisterSparseTensor(C.byref(t.address)) == 0
+    try:
+        for mode in range(3):
+            f64 = oracle.mttkrp(g["in_inds"], g["in_vals"], facs, mode, kind="f64")
+            assert rel_fro(h.MTTKRP(mode, dense_mats(facs, 100)), f64) <= TOL
+            assert rel_fro(t.MTTKRP(mode, dense_mats(facs, 100)), f64) <= TOL
+    finally:
+        assert lib().ptiUnregisterHiCOO(C.byref(h.address)) == 0
+        assert lib().ptiUnregisterSparseTensor(C.byref(t.address)) == 0
+
+
+def test_device_api_fp32_and_bf16(oracle):
+    import torch
+    from hiparti_amd.device import DevCOO, DevHiCOO
+    ndims = [5000, 4000, 300]
+    R = 32
+    t = COOTensor.synthetic(ndims, 200000, dist=[1, 1, 1], zipf_s=[1.1] * 3, permute=[1, 1, 1], seed=3)
+    inds, vals = t.inds(), t.values()
+    rng = np.random.default_rng(3)
+    facs = [rng.random((d, R)).astype(np.float32) for d in ndims]
+    dfacs = [torch.from_numpy(f).cuda() for f in facs]
+    dc = DevCOO(t)
+    h = t.toHiCOO(7, 14, 14)
+    dh = DevHiCOO(h)
+    assert dh.nnz == 200000
+    for mode in range(3):
+        f64 = oracle.mttkrp(inds, vals, facs, mode, kind="f64")
+        out = dh.mttkrp(dfacs, mode)
+        torch.cuda.synchronize()
+        assert rel_fro(out.cpu().numpy()[:, :R], f64) <= TOL
+        out = dc.mttkrp(dfacs, mode)
+        torch.cuda.synchronize()
+        assert rel_fro(out.cpu().numpy()[:, :R], f64) <= TOL
+        # BF16 factors, FP32 products and accumulation: compare with the FP64
+        # oracle on the same rounded factors.
+        bf = [f.to(torch.bfloat16) for f in dfacs]
+        rounded = [b.float().cpu().numpy() for b in bf]
+        out = dh.mttkrp_bf16(bf, mode)
+        torch.cuda.synchronize()
+        f64r = oracle.mttkrp(inds, vals, rounded, mode, kind="f64")
+        assert rel_fro(out.cpu().numpy()[:, :R], f64r) <= TOL
+        # and its distance to the FP32-factor result is the bf16 rounding (< 1%)
+        assert rel_fro(out.cpu().numpy()[:, :R], f64) < 1e-2
+
+
+def test_large_zipf_properties(oracle):
+    """Size-independent checks on a c1-shaped tensor (Zipf 1.1, relabelled
+    modes, 2^20 x 2^20 x 2^16, sb=7 sk=14) at 4M nonzeros: all-ones factors
+    give segmented sums; HiCOO and COO paths agree with the FP64 oracle."""
+    ndims = [1 << 20, 1 << 20, 1 << 16]
+    t = COOTensor.synthetic(ndims, 4_000_000, dist=[1, 1, 1], zipf_s=[1.1] * 3, permute=[1, 1, 1], seed=42)
+    inds, vals = t.inds(), t.values()
+    R = 32
+    h = t.toHiCOO(7, 14, 14)
+    ones = [np.ones((d, R), np.float32) for d in ndims]
+    rng = np.random.default_rng(9)
+    facs = [rng.random((d, R)).astype(np.float32) for d in ndims]
+    for mode in range(3):
+        seg = np.bincount(inds[mode], weights=vals.astype(np.float64), minlength=ndims[mode])
+        out = h.MTTKRP(mode, dense_mats(ones, max(ndims)))
+        assert rel_fro(out, np.repeat(seg[:, None], R, axis=1)) <= TOL
+        f64 = oracle.mttkrp(inds, vals, facs, mode, kind="f64")
+        assert rel_fro(h.MTTKRP(mode, dense_mats(facs, max(ndims))), f64) <= TOL
+        assert rel_fro(t.MTTKRP(mode, dense_mats(facs, max(ndims))), f64) <= TOL
