@@ -25,12 +25,30 @@ class PtiError(RuntimeError):
         super().__init__(msg)
 
 
+def _share_torch_hip_runtime():
+    """Load torch's HIP runtime before libHiParTI.so when torch is present.
+
+    torch's libraries request the runtime by the unversioned name
+    libamdhip64.so from their own directory, while libHiParTI.so requests
+    libamdhip64.so.7.  If libHiParTI.so were loaded first, torch would later
+    load a second copy of the runtime, and one process would then hold two HIP
+    runtimes (torch then sees no device).  Loaded in the other order, the
+    dynamic loader resolves libHiParTI.so's libamdhip64.so.7 to torch's copy by
+    its soname, so both share one runtime, one device context and one set of
+    streams."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError("libHiParTI.so not found at %s; build it with "
                               "`python -m hiparti_amd.build`" % LIB_PATH)
+        _share_torch_hip_runtime()
         _lib = _ffi.declare(ctypes.CDLL(LIB_PATH))
     return _lib
 

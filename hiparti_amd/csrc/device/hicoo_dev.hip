@@ -1,51 +1,72 @@
 // HiCOO tensors resident in HBM and the HiCOO MTTKRP entry points.
 //
-// Device layout (built once by ptiDevHiCOOUpload from the host HiCOO arrays):
-//   bptr      u32[nb + 1]        nonzero offset of each block (nnz < 2^32)
-//   binds     u32[nb * N]        block coordinates, block-major (one 16-byte
-//                                read per block for N = 3 with bptr)
-//   einds     u32[nnz * EW]      the N 8-bit element offsets of a nonzero
-//                                packed into EW = ceil(N/4) words
-//   vals      f32[nnz]
-//   chunk_blk u32[nchunks + 1]   block holding nonzero c * 256; last = nb
-//   perm[m]   u8[nchunks * 256]  per mode: each 256-nonzero chunk's local
-//                                indices sorted by the mode-m coordinate
-// The host HiCOO struct itself is left untouched.
+// Device layout.  For every target mode m the upload keeps one copy of the
+// HiCOO blocks, stably reordered so that blocks with the same mode-m block
+// coordinate (a "block-row": 2^sb_bits output rows) are contiguous.  Blocks,
+// their 8-bit element offsets and values are the host tensor's, unchanged;
+// only the block order differs per mode.  Per mode:
+//   bptr       u32[nb + 1]       nonzero offsets of the reordered blocks
+//   binds      u32[nb * N]       block coordinates, block-major
+//   einds      u32[nnz * EW]     the N element offsets of a nonzero packed
+//                                into EW = ceil(N/4) words
+//   vals       f32[nnz]
+//   chunk_z    u32[nchunks + 1]  chunks of <= 256 nonzeros that never cross
+//   chunk_blk  u32[nchunks + 1]  a block-row; first nonzero and its block
+//   unit_chunk u32[nunits + 1]   units: runs of <= UNIT_CHUNKS chunks of one
+//   unit_br    u32[nunits]       block-row, each processed by one workgroup
+//   unit_excl  u32[nunits]       1 if the block-row has a single unit
+// N copies cost N times the memory of one (6.8 GB for a 100M-nonzero order-3
+// tensor with 73M blocks), which HBM holds easily; in exchange each MTTKRP
+// streams its copy sequentially and accumulates outputs in LDS.
+// The host HiCOO struct is not referenced after the upload.
 #include <atomic>
 #include <mutex>
 #include <unordered_map>
 
 #include "../../../include/hiparti_dev.h"
+#include "../host/sorting.h"
 #include "dropin.hpp"
 #include "launch.hpp"
+
+namespace {
+constexpr uint32_t UNIT_CHUNKS = 64;  // chunks (of 256 nonzeros) per workgroup unit
+}
+
+struct ptiDevHiCOOMode {
+    pti::dev::Buffer bptr, binds, einds, vals, chunk_z, chunk_blk, unit_chunk, unit_br, unit_excl;
+    uint32_t nchunks = 0, nunits = 0;
+    bool ready = false;
+};
 
 struct ptiDevHiCOO {
     uint32_t nmodes = 0;
     uint32_t sb_bits = 0;
     uint64_t nnz = 0;
     uint64_t nb = 0;
-    uint32_t nchunks = 0;
     ptiIndex ndims[8] = {0};
-    pti::dev::Buffer bptr, binds, einds, vals, chunk_blk;
-    pti::dev::Buffer perm[8];
+    ptiDevHiCOOMode mc[8];
     uint64_t bytes = 0;
 };
 
 namespace pti {
 namespace dev {
 
-HiCOOArgs hicoo_args(const ptiDevHiCOO *t, int mode_perm) {
+HiCOOArgs hicoo_args(const ptiDevHiCOO *t, uint32_t mode) {
+    const ptiDevHiCOOMode &c = t->mc[mode];
     HiCOOArgs h;
-    h.bptr = t->bptr.as<uint32_t>();
-    h.binds = t->binds.as<uint32_t>();
-    h.einds = t->einds.as<uint32_t>();
-    h.vals = t->vals.as<float>();
-    h.chunk_blk = t->chunk_blk.as<uint32_t>();
-    h.perm = mode_perm >= 0 ? t->perm[mode_perm].as<uint8_t>() : nullptr;
-    h.nnz = t->nnz;
+    h.bptr = c.bptr.as<uint32_t>();
+    h.binds = c.binds.as<uint32_t>();
+    h.einds = c.einds.as<uint32_t>();
+    h.vals = c.vals.as<float>();
+    h.chunk_z = c.chunk_z.as<uint32_t>();
+    h.chunk_blk = c.chunk_blk.as<uint32_t>();
+    h.unit_chunk = c.unit_chunk.as<uint32_t>();
+    h.unit_br = c.unit_br.as<uint32_t>();
+    h.unit_excl = c.unit_excl.as<uint32_t>();
     h.nb = (uint32_t)t->nb;
-    h.nchunks = t->nchunks;
+    h.nunits = c.nunits;
     h.sb_bits = t->sb_bits;
+    h.nrows = t->ndims[mode];
     return h;
 }
 
@@ -54,22 +75,21 @@ namespace {
 std::mutex g_reg_mu;
 std::unordered_map<const void *, ptiDevHiCOO *> g_hicoo_reg;
 
-int upload(ptiDevHiCOO **out, const ptiSparseTensorHiCOO *hitsr, hipStream_t s) {
-    static const char *M = "DevHiCOO Upload";
-    PTI_CHECK(require_device(M), M);
+int validate(const ptiSparseTensorHiCOO *hitsr, const char *M) {
     const uint32_t N = hitsr->nmodes;
     if (N < 2 || N > 8) PTI_FAIL(PTIERR_VALUE_ERROR, M, "nmodes must be in [2, 8]");
+    if (hitsr->sb_bits > 7) PTI_FAIL(PTIERR_VALUE_ERROR, M, "sb_bits must be <= 7");
     const uint64_t nnz = hitsr->nnz;
     const uint64_t nb = hitsr->bptr.len ? hitsr->bptr.len - 1 : 0;
-    if (nnz >= (1ull << 32)) PTI_FAIL(PTIERR_VALUE_ERROR, M, "nnz must be < 2^32");
+    if (nnz >= (1ull << 32) || nb >= (1ull << 32)) PTI_FAIL(PTIERR_VALUE_ERROR, M, "nnz must be < 2^32");
     if (hitsr->values.len < nnz || (nnz && nb == 0)) PTI_FAIL(PTIERR_VALUE_ERROR, M, "inconsistent HiCOO arrays");
     for (uint32_t m = 0; m < N; ++m)
         if (hitsr->einds[m].len < nnz || hitsr->binds[m].len < nb)
             PTI_FAIL(PTIERR_VALUE_ERROR, M, "inconsistent HiCOO arrays");
     if (nb && hitsr->bptr.data[nb] != nnz) PTI_FAIL(PTIERR_VALUE_ERROR, M, "bptr does not end at nnz");
-    // The kernels index factor rows and LDS staging buffers with these
-    // arrays: every block must be non-empty (a 256-nonzero chunk then spans
-    // at most 257 blocks) and every coordinate must lie inside its mode.
+    // The kernels index factor rows, output rows and LDS staging buffers with
+    // these arrays: every block must be non-empty (a 256-nonzero chunk then
+    // spans at most 257 blocks) and every coordinate must lie inside its mode.
     if (nnz) {
         std::atomic<bool> bad(false);
         pti::parallel_for(nb, 0, [&](uint64_t b, uint64_t e, int) {
@@ -88,39 +108,82 @@ int upload(ptiDevHiCOO **out, const ptiSparseTensorHiCOO *hitsr, hipStream_t s) 
         });
         if (bad.load()) PTI_FAIL(PTIERR_VALUE_ERROR, M, "empty block or coordinate outside its mode");
     }
+    return 0;
+}
 
-    ptiDevHiCOO *t = new ptiDevHiCOO();
-    t->nmodes = N;
-    t->sb_bits = hitsr->sb_bits;
-    t->nnz = nnz;
-    t->nb = nb;
-    for (uint32_t m = 0; m < N; ++m) t->ndims[m] = hitsr->ndims[m];
+// Builds and uploads the copy of target mode `mode`.
+int build_mode(ptiDevHiCOO *t, const ptiSparseTensorHiCOO *hitsr, uint32_t mode, hipStream_t s, const char *M) {
+    const uint32_t N = t->nmodes;
+    const uint64_t nnz = t->nnz, nb = t->nb;
     const int EW = (N + 3) / 4;
-    const uint64_t nchunks = (nnz + CHUNK - 1) / CHUNK;
-    t->nchunks = (uint32_t)nchunks;
+    ptiDevHiCOOMode &c = t->mc[mode];
 
-    // Host-side repacking.
-    std::vector<uint32_t> hb(nb + 1), hbi(nb * N), he(nnz * EW, 0u), hc(nchunks + 1);
-    pti::parallel_for(nb + 1, 0, [&](uint64_t b, uint64_t e, int) {
-        for (uint64_t x = b; x < e; ++x) hb[x] = (uint32_t)hitsr->bptr.data[x];
-    });
-    pti::parallel_for(nb, 0, [&](uint64_t b, uint64_t e, int) {
-        for (uint64_t x = b; x < e; ++x)
-            for (uint32_t m = 0; m < N; ++m) hbi[x * N + m] = hitsr->binds[m].data[x];
-    });
-    pti::parallel_for(nnz, 0, [&](uint64_t b, uint64_t e, int) {
-        for (uint64_t z = b; z < e; ++z)
-            for (uint32_t m = 0; m < N; ++m) he[z * EW + (m >> 2)] |= (uint32_t)hitsr->einds[m].data[z] << ((m & 3) * 8);
-    });
-    {
-        uint64_t blk = 0;
-        for (uint64_t c = 0; c < nchunks; ++c) {
-            const uint64_t z = c * CHUNK;
-            while (hitsr->bptr.data[blk + 1] <= z) ++blk;
-            hc[c] = (uint32_t)blk;
-        }
-        hc[nchunks] = (uint32_t)nb;
+    // Stable order of the blocks by their mode-m block coordinate.
+    std::vector<uint32_t> key(nb), ids(nb);
+    uint32_t kmax = 0;
+    for (uint64_t b = 0; b < nb; ++b) {
+        key[b] = hitsr->binds[mode].data[b];
+        ids[b] = (uint32_t)b;
+        kmax = std::max(kmax, key[b]);
     }
+    int kbits = 0;
+    while (kbits < 32 && (kmax >> kbits)) ++kbits;
+    pti::radix_sort_pairs<uint32_t>(key, ids, kbits, 0);
+
+    // Reordered blocks.
+    std::vector<uint32_t> hb(nb + 1), hbi(nb * N);
+    hb[0] = 0;
+    for (uint64_t i = 0; i < nb; ++i) {
+        const uint64_t b = ids[i];
+        hb[i + 1] = hb[i] + (uint32_t)(hitsr->bptr.data[b + 1] - hitsr->bptr.data[b]);
+    }
+    std::vector<uint32_t> he(nnz * EW, 0u);
+    std::vector<float> hv(nnz);
+    pti::parallel_for(nb, 0, [&](uint64_t lo, uint64_t hi, int) {
+        for (uint64_t i = lo; i < hi; ++i) {
+            const uint64_t b = ids[i];
+            for (uint32_t m = 0; m < N; ++m) hbi[i * N + m] = hitsr->binds[m].data[b];
+            const uint64_t src = hitsr->bptr.data[b], len = hitsr->bptr.data[b + 1] - src;
+            for (uint64_t j = 0; j < len; ++j) {
+                const uint64_t z = hb[i] + j;
+                for (uint32_t m = 0; m < N; ++m)
+                    he[z * EW + (m >> 2)] |= (uint32_t)hitsr->einds[m].data[src + j] << ((m & 3) * 8);
+                hv[z] = hitsr->values.data[src + j];
+            }
+        }
+    });
+
+    // Chunks (never crossing a block-row) and units.
+    std::vector<uint32_t> cz, cb, uc, ubr, uex;
+    cz.reserve(nnz / pti::dev::CHUNK + 1024);
+    cb.reserve(nnz / pti::dev::CHUNK + 1024);
+    for (uint64_t i = 0; i < nb;) {
+        const uint32_t br = key[i];
+        uint64_t j = i;
+        while (j < nb && key[j] == br) ++j;
+        const uint64_t gz1 = hb[j];
+        const uint64_t first = cz.size();
+        uint64_t blk = i;
+        for (uint64_t z = hb[i]; z < gz1; z += pti::dev::CHUNK) {
+            while (hb[blk + 1] <= z) ++blk;
+            cz.push_back((uint32_t)z);
+            cb.push_back((uint32_t)blk);
+        }
+        const uint64_t nch = cz.size() - first;
+        const uint64_t nu = (nch + UNIT_CHUNKS - 1) / UNIT_CHUNKS;
+        for (uint64_t u = 0; u < nu; ++u) {
+            uc.push_back((uint32_t)(first + u * nch / nu));
+            ubr.push_back(br);
+            uex.push_back(nu == 1 ? 1u : 0u);
+        }
+        i = j;
+    }
+    c.nchunks = (uint32_t)cz.size();
+    c.nunits = (uint32_t)ubr.size();
+    cz.push_back((uint32_t)nnz);
+    cb.push_back((uint32_t)nb);
+    uc.push_back(c.nchunks);
+
     auto put = [&](Buffer &buf, const void *src, size_t bytes) -> int {
         PTI_HIP(buf.alloc(bytes), M);
         if (bytes) PTI_HIP(hipMemcpyAsync(buf.ptr, src, bytes, hipMemcpyHostToDevice, s), M);
@@ -128,34 +191,43 @@ int upload(ptiDevHiCOO **out, const ptiSparseTensorHiCOO *hitsr, hipStream_t s) 
         return 0;
     };
     int rc = 0;
-    if (!rc) rc = put(t->bptr, hb.data(), hb.size() * 4);
-    if (!rc) rc = put(t->binds, hbi.data(), hbi.size() * 4);
-    if (!rc) rc = put(t->einds, he.data(), he.size() * 4);
-    if (!rc) rc = put(t->vals, hitsr->values.data, nnz * 4);
-    if (!rc) rc = put(t->chunk_blk, hc.data(), hc.size() * 4);
-    if (!rc) {
-        HiCOOArgs h = hicoo_args(t, -1);
-        for (uint32_t m = 0; m < N && !rc; ++m) {
-            hipError_t e = t->perm[m].alloc(nchunks * CHUNK);
-            if (e == hipSuccess) e = launch_sort_chunks(N, h, m, t->perm[m].as<uint8_t>(), s);
-            if (e != hipSuccess) {
-                pti_ComplainError(M, hip_code(e), __FILE__, __LINE__, hipGetErrorString(e));
-                rc = hip_code(e);
-            }
-            t->bytes += nchunks * CHUNK;
-        }
+    if (!rc) rc = put(c.bptr, hb.data(), hb.size() * 4);
+    if (!rc) rc = put(c.binds, hbi.data(), hbi.size() * 4);
+    if (!rc) rc = put(c.einds, he.data(), he.size() * 4);
+    if (!rc) rc = put(c.vals, hv.data(), hv.size() * 4);
+    if (!rc) rc = put(c.chunk_z, cz.data(), cz.size() * 4);
+    if (!rc) rc = put(c.chunk_blk, cb.data(), cb.size() * 4);
+    if (!rc) rc = put(c.unit_chunk, uc.data(), uc.size() * 4);
+    if (!rc) rc = put(c.unit_br, ubr.data(), ubr.size() * 4);
+    if (!rc) rc = put(c.unit_excl, uex.data(), uex.size() * 4);
+    // The copies read the host vectors above: finish them before they go.
+    hipError_t e = hipStreamSynchronize(s);
+    if (!rc && e != hipSuccess) {
+        pti_ComplainError(M, hip_code(e), __FILE__, __LINE__, hipGetErrorString(e));
+        rc = hip_code(e);
     }
-    if (!rc) {
-        hipError_t e = hipStreamSynchronize(s);
-        if (e != hipSuccess) {
-            pti_ComplainError(M, hip_code(e), __FILE__, __LINE__, hipGetErrorString(e));
-            rc = hip_code(e);
+    if (!rc) c.ready = true;
+    return rc;
+}
+
+// Upload the copies of the modes in `mode_mask` (bit m = target mode m).
+int upload(ptiDevHiCOO **out, const ptiSparseTensorHiCOO *hitsr, uint32_t mode_mask, hipStream_t s) {
+    static const char *M = "DevHiCOO Upload";
+    PTI_CHECK(require_device(M), M);
+    PTI_CHECK(validate(hitsr, M), M);
+    ptiDevHiCOO *t = new ptiDevHiCOO();
+    t->nmodes = hitsr->nmodes;
+    t->sb_bits = hitsr->sb_bits;
+    t->nnz = hitsr->nnz;
+    t->nb = hitsr->nnz ? hitsr->bptr.len - 1 : 0;
+    for (uint32_t m = 0; m < t->nmodes; ++m) t->ndims[m] = hitsr->ndims[m];
+    for (uint32_t m = 0; m < t->nmodes; ++m) {
+        if (!(mode_mask >> m & 1u)) continue;
+        int rc = build_mode(t, hitsr, m, s, M);
+        if (rc) {
+            delete t;
+            return rc;
         }
-    }
-    if (rc) {
-        (void)hipStreamSynchronize(s);  // pending copies read the host staging vectors
-        delete t;
-        return rc;
     }
     *out = t;
     return 0;
@@ -164,7 +236,7 @@ int upload(ptiDevHiCOO **out, const ptiSparseTensorHiCOO *hitsr, hipStream_t s) 
 }  // namespace
 
 // Shared by the FP32 and BF16 entry points: zero the output rows, then run the
-// kernel once per 64*V-rank column tile.
+// tile kernel once per 64-rank column tile.
 int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiIndex R, ptiIndex stride,
               ptiIndex out_stride, const ptiIndex mats_order[], ptiIndex mode, bool bf16, hipStream_t s,
               const char *M) {
@@ -173,15 +245,20 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
     if (R == 0 || stride < R || out_stride < R) PTI_FAIL(PTIERR_SHAPE_MISMATCH, M, "bad rank or stride");
     for (uint32_t i = 1; i < N; ++i)
         if (mats_order[i] >= N) PTI_FAIL(PTIERR_VALUE_ERROR, M, "mats_order entry out of range");
+    if (t->nnz && !t->mc[mode].ready) PTI_FAIL(PTIERR_VALUE_ERROR, M, "mode was not uploaded");
     PTI_HIP(zero_floats(out, (uint64_t)t->ndims[mode] * out_stride, s), M);
     if (t->nnz == 0) return 0;
-    const ptiIndex tile = 256;
+    // 64 ranks per LDS tile; 32 for orders >= 6, whose larger staging area
+    // would otherwise push a workgroup past 64 KB of LDS.
+    const ptiIndex tile = N >= 6 ? 32 : 64;
+    const size_t esz = bf16 ? 2 : 4;
     for (ptiIndex c0 = 0; c0 < R; c0 += tile) {
         const ptiIndex Rt = std::min<ptiIndex>(tile, R - c0);
+        // Column tiles start at multiples of 32 elements, so they keep the
+        // alignment the vector width V relies on.
         Shape sh;
         if (!choose_shape(Rt, stride, out_stride, sh)) PTI_FAIL(PTIERR_VALUE_ERROR, M, "unsupported rank tile");
         FactorArgs a;
-        const size_t esz = bf16 ? 2 : 4;
         for (uint32_t m = 0; m < N; ++m) a.fac[m] = static_cast<const char *>(facs[m]) + (size_t)c0 * esz;
         a.out = out + c0;
         a.R = Rt;
@@ -189,7 +266,7 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
         a.out_stride = out_stride;
         a.mode = mode;
         for (uint32_t i = 0; i < N; ++i) a.order[i] = mats_order[i];
-        PTI_HIP(launch_hicoo(N, hicoo_args(t, (int)mode), a, sh, bf16, s), M);
+        PTI_HIP(launch_hicoo(N, hicoo_args(t, mode), a, sh, bf16, s), M);
     }
     return 0;
 }
@@ -202,7 +279,7 @@ using namespace pti::dev;
 extern "C" {
 
 int ptiDevHiCOOUpload(ptiDevHiCOO **out, ptiSparseTensorHiCOO const *hitsr, void *stream) {
-    return upload(out, hitsr, static_cast<hipStream_t>(stream));
+    return upload(out, hitsr, 0xFFu, static_cast<hipStream_t>(stream));
 }
 
 void ptiDevHiCOOFree(ptiDevHiCOO *t) { delete t; }
@@ -234,7 +311,7 @@ int ptiDevMTTKRPHiCOO_BF16(ptiDevHiCOO const *t, uint16_t const *const dfacs[], 
 
 int ptiRegisterHiCOO(ptiSparseTensorHiCOO const *hitsr) {
     ptiDevHiCOO *t = nullptr;
-    PTI_CHECK(upload(&t, hitsr, nullptr), "Register HiCOO");
+    PTI_CHECK(upload(&t, hitsr, 0xFFu, nullptr), "Register HiCOO");
     std::lock_guard<std::mutex> lk(g_reg_mu);
     auto it = g_hicoo_reg.find(hitsr);
     if (it != g_hicoo_reg.end()) delete it->second;
@@ -266,7 +343,9 @@ int ptiMTTKRPHiCOO(ptiSparseTensorHiCOO const *const hitsr, ptiMatrix *mats[], p
         if (it != g_hicoo_reg.end()) t = it->second;
     }
     if (!t) {
-        PTI_CHECK(upload(&owned, hitsr, s), M);
+        // Unregistered tensors are uploaded for this call only, and only the
+        // copy of the requested mode is built.
+        PTI_CHECK(upload(&owned, hitsr, 1u << mode, s), M);
         t = owned;
     }
     StagedFactors sf;

@@ -18,7 +18,8 @@ struct Shape {
 
 // V = 4 when R and every stride are multiples of 4 (16-byte row pieces),
 // else 2, else 1; G = ceil(R / V) rounded up to a power of two in [4, 64].
-// Returns false when R exceeds 64 * V (callers split the rank into 256-column tiles).
+// Returns false when R exceeds 64 * V; callers split larger ranks into
+// column tiles (256 columns for COO, 64 for the HiCOO LDS tile).
 inline bool choose_shape(uint32_t R, uint32_t stride, uint32_t out_stride, Shape &sh) {
     if (R == 0) return false;
     int V = 1;
@@ -37,8 +38,7 @@ inline bool choose_shape(uint32_t R, uint32_t stride, uint32_t out_stride, Shape
 
 #define PTI_DECLARE_LAUNCHERS(N)                                                                       \
     hipError_t launch_hicoo_##N(const HiCOOArgs &h, const FactorArgs &a, Shape sh, bool bf16, hipStream_t s); \
-    hipError_t launch_coo_##N(const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s);        \
-    hipError_t launch_sort_chunks_##N(const HiCOOArgs &h, uint32_t mode, uint8_t *perm, hipStream_t s);
+    hipError_t launch_coo_##N(const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s);
 
 PTI_DECLARE_LAUNCHERS(2)
 PTI_DECLARE_LAUNCHERS(3)
@@ -69,11 +69,6 @@ inline hipError_t launch_hicoo(uint32_t N, const HiCOOArgs &h, const FactorArgs 
 }
 inline hipError_t launch_coo(uint32_t N, const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s) {
 #define PTI_CALL(n) launch_coo_##n(c, a, sh, s)
-    PTI_DISPATCH_N(N, PTI_CALL)
-#undef PTI_CALL
-}
-inline hipError_t launch_sort_chunks(uint32_t N, const HiCOOArgs &h, uint32_t mode, uint8_t *perm, hipStream_t s) {
-#define PTI_CALL(n) launch_sort_chunks_##n(h, mode, perm, s)
     PTI_DISPATCH_N(N, PTI_CALL)
 #undef PTI_CALL
 }

@@ -22,13 +22,18 @@ inline uint32_t grid_for_chunks(uint32_t nchunks) {
 
 template <int V, int G>
 hipError_t hicoo_vg(const HiCOOArgs &h, const FactorArgs &a, bool bf16, hipStream_t s) {
-    if (h.nchunks == 0) return hipSuccess;
-    dim3 grid(grid_for_chunks(h.nchunks)), block(256);
-    if (bf16)
-        hipLaunchKernelGGL((hicoo_mttkrp_kernel<N, V, G, uint16_t>), grid, block, 0, s, h, a);
-    else
-        hipLaunchKernelGGL((hicoo_mttkrp_kernel<N, V, G, float>), grid, block, 0, s, h, a);
-    return hipGetLastError();
+    if constexpr (G * V > 64) {
+        return hipErrorInvalidValue;  // the LDS tile holds at most 64 ranks
+    } else {
+        if (h.nunits == 0) return hipSuccess;
+        const size_t lds = (size_t)(1u << h.sb_bits) * (G * V + 1) * sizeof(float);
+        dim3 grid(h.nunits), block(256);
+        if (bf16)
+            hipLaunchKernelGGL((hicoo_tile_kernel<N, V, G, uint16_t>), grid, block, lds, s, h, a);
+        else
+            hipLaunchKernelGGL((hicoo_tile_kernel<N, V, G, float>), grid, block, lds, s, h, a);
+        return hipGetLastError();
+    }
 }
 
 template <int V, int G>
@@ -81,13 +86,6 @@ hipError_t PTI_CAT(launch_coo_, PTI_NMODES)(const COOArgs &c, const FactorArgs &
         case 1: return coo_v<1>(c, a, sh.G, s);
         default: return hipErrorInvalidValue;
     }
-}
-
-hipError_t PTI_CAT(launch_sort_chunks_, PTI_NMODES)(const HiCOOArgs &h, uint32_t mode, uint8_t *perm,
-                                                    hipStream_t s) {
-    if (h.nchunks == 0) return hipSuccess;
-    hipLaunchKernelGGL((hicoo_sort_chunks_kernel<N>), dim3(h.nchunks), dim3(256), 0, s, h, mode, perm);
-    return hipGetLastError();
 }
 
 }  // namespace dev

@@ -1,32 +1,36 @@
 // MTTKRP kernels for gfx950 (wave64).
 //
-// Work decomposition.  The nonzeros of the tensor (in storage order) are cut
-// into chunks of CHUNK = 256 consecutive nonzeros; one wavefront owns one
-// chunk, a 256-thread workgroup owns four.  Workgroup ids are remapped so
-// that consecutive chunks land on the same XCD (the hardware deals
-// workgroups to the 8 XCDs round-robin), which keeps the factor and output
-// rows of neighbouring chunks in one L2.
+// Common pieces.  Nonzeros are processed in chunks of at most CHUNK = 256
+// consecutive nonzeros; a wavefront decodes a chunk into LDS (coordinates of
+// every mode and the values) and then walks it with the 64 lanes split into
+// NG = 64/G groups of G lanes: a group handles one nonzero per step and each
+// lane V consecutive ranks (float4 / float2 / float loads of the factor
+// rows, or 4/2/1 bf16 values for the BF16 form).  The contribution
+// v * prod_k A_k[i_k][r] is formed in mats_order order, in FP32.
 //
-// 1. Staging.  The wave decodes its chunk into LDS: full coordinates of every
-//    nonzero in every mode and the values.  For HiCOO the block of nonzero z
-//    is found by binary search in the chunk's slice of bptr (held in LDS) and
-//    the coordinate is (binds << sb_bits) + eind; for COO the coordinates are
-//    read directly.
-// 2. Processing order.  The chunk is walked in the order of an optional
-//    per-mode permutation perm (1 byte per nonzero) that sorts the chunk by
-//    output row; the COO path uses per-mode copies already sorted by the
-//    output index, so it walks the chunk in storage order.
-// 3. Arithmetic.  The 64 lanes are split into NG = 64/G groups of G lanes; a
-//    group handles one nonzero per step and each lane V consecutive ranks
-//    (float4 / float2 / float loads of the factor rows, or 4/2/1 bf16 values
-//    for the BF16 form).  The contribution v * prod_k A_k[i_k][r] is formed in
-//    mats_order order, in FP32.
-// 4. Reduction.  Within a step the NG contributions are combined by a
-//    segmented inclusive scan over groups keyed on the output row; the last
-//    group of a segment adds the segment sum to the output with one FP32
-//    atomic per rank, except that a segment still open at the end of a step is
-//    carried into the next step.  Each output row therefore receives one
-//    atomic per (chunk, row) run instead of one per nonzero.
+// HiCOO kernel (hicoo_tile_kernel).  The device keeps, for each target mode
+// m, a copy of the HiCOO blocks grouped by their mode-m block coordinate
+// (block-row); blocks, 8-bit element offsets and values are unchanged.  A
+// workgroup owns one "unit": a run of chunks inside one block-row.  All
+// nonzeros of a unit write output rows (block-row << sb_bits) + eind_m, i.e.
+// 2^sb_bits rows, which the workgroup accumulates in an LDS tile indexed by
+// the 8-bit element offset.  At the end the tile is written once: with plain
+// stores when the unit is the block-row's only unit, otherwise with one FP32
+// atomic per element.  Global atomics thus scale with the number of units,
+// not with the number of nonzeros (a Zipf head row no longer serialises
+// millions of atomics).
+//
+// COO kernel (coo_mttkrp_kernel).  Per target mode the device keeps a copy
+// sorted by the output index.  Within a step the NG contributions are
+// combined by a segmented inclusive scan over groups keyed on the output row;
+// the last group of a segment adds the segment sum to the output with one
+// FP32 atomic per rank, and a segment still open at the end of a step is
+// carried into the next step, so each output row receives one atomic per
+// (chunk, row) run.
+//
+// Workgroup ids are remapped so that consecutive work lands on the same XCD
+// (the hardware deals workgroups to the 8 XCDs round-robin), which keeps the
+// rows used by neighbouring work in one L2.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -50,17 +54,21 @@ struct FactorArgs {
     uint32_t order[MAX_MODES];   // order[1..N-1]: factor modes in product order
 };
 
+// One target mode's copy of a HiCOO tensor (see the header comment).
 struct HiCOOArgs {
-    const uint32_t *bptr;       // nb + 1 nonzero offsets (32-bit)
-    const uint32_t *binds;      // nb * N block coordinates, block-major
-    const uint32_t *einds;      // nnz * EW packed 8-bit element offsets
-    const float *vals;          // nnz values
-    const uint32_t *chunk_blk;  // nchunks + 1: block holding nonzero c*CHUNK; [nchunks] = nb
-    const uint8_t *perm;        // nullable: per-chunk row-sorted order for the target mode
-    uint64_t nnz;
+    const uint32_t *bptr;        // nb + 1 nonzero offsets (32-bit)
+    const uint32_t *binds;       // nb * N block coordinates, block-major
+    const uint32_t *einds;       // nnz * EW packed 8-bit element offsets
+    const float *vals;           // nnz values
+    const uint32_t *chunk_z;     // nchunks + 1: first nonzero of each chunk; [nchunks] = nnz
+    const uint32_t *chunk_blk;   // nchunks + 1: block holding chunk_z[c]; [nchunks] = nb
+    const uint32_t *unit_chunk;  // nunits + 1: first chunk of each unit
+    const uint32_t *unit_br;     // nunits: block-row (mode-m block coordinate) of each unit
+    const uint32_t *unit_excl;   // nunits: 1 if the unit is its block-row's only unit
     uint32_t nb;
-    uint32_t nchunks;
+    uint32_t nunits;
     uint32_t sb_bits;
+    uint32_t nrows;              // ndims of the target mode
 };
 
 struct COOArgs {
@@ -134,24 +142,30 @@ struct Staging {
     uint32_t bptr[CHUNK + 4];
 };
 
-// Decode chunk `chunk` of a HiCOO tensor into st (one wave, 64 lanes).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Decode chunk `chunk` of a HiCOO mode copy into st (one wave, 64 lanes).
+// A chunk holds at most CHUNK nonzeros, all blocks non-empty, so it spans at
+// most CHUNK + 1 blocks and bptr[b0 .. blast] fits in st.bptr.
 // Returns the number of nonzeros in the chunk.
 template <int N>
 __device__ __forceinline__ uint32_t stage_hicoo(const HiCOOArgs &h, uint32_t chunk, Staging<N> &st, int lane) {
     constexpr int EW = (N + 3) / 4;
-    const uint64_t z0 = (uint64_t)chunk * CHUNK;
-    const uint64_t rem = h.nnz - z0;
-    const uint32_t cnt = rem < CHUNK ? (uint32_t)rem : (uint32_t)CHUNK;
+    const uint32_t z0 = h.chunk_z[chunk];
+    const uint32_t cnt = h.chunk_z[chunk + 1] - z0;
     const uint32_t b0 = h.chunk_blk[chunk];
     const uint32_t b1 = h.chunk_blk[chunk + 1];
     const uint32_t blast = (b1 + 1 < h.nb) ? b1 + 1 : h.nb;   // inclusive bptr index
     const uint32_t nbp = blast - b0 + 1;                      // <= CHUNK + 2
+    wave_sync();
     for (uint32_t j = lane; j < nbp; j += 64) st.bptr[j] = h.bptr[b0 + j];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_sync();
     for (uint32_t zl = lane; zl < cnt; zl += 64) {
-        const uint32_t z = (uint32_t)(z0 + zl);
+        const uint32_t z = z0 + zl;
         // Largest j with bptr[b0 + j] <= z, j in [0, nbp - 2].
         uint32_t lo = 0, hi = nbp - 2;
         while (lo < hi) {
@@ -169,9 +183,7 @@ __device__ __forceinline__ uint32_t stage_hicoo(const HiCOOArgs &h, uint32_t chu
         }
         st.val[zl] = h.vals[z];
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_sync();
     return cnt;
 }
 
@@ -180,25 +192,25 @@ __device__ __forceinline__ uint32_t stage_coo(const COOArgs &c, uint32_t chunk, 
     const uint64_t z0 = (uint64_t)chunk * CHUNK;
     const uint64_t rem = c.nnz - z0;
     const uint32_t cnt = rem < CHUNK ? (uint32_t)rem : (uint32_t)CHUNK;
+    wave_sync();
     for (uint32_t zl = lane; zl < cnt; zl += 64) {
 #pragma unroll
         for (int m = 0; m < N; ++m) st.coord[m][zl] = c.inds[m][z0 + zl];
         st.val[zl] = c.vals[z0 + zl];
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_sync();
     return cnt;
 }
 
-// Contribution of sorted position p of the staged chunk for this lane's V ranks.
+// Contribution of position p of the staged chunk for this lane's V ranks;
+// returns the output row (INVALID_ROW past the end of the chunk).
 template <int N, int V, typename FT>
-__device__ __forceinline__ uint32_t contribution(const FactorArgs &a, const Staging<N> &st, const uint8_t *perm,
-                                                 uint32_t p, uint32_t cnt, uint32_t col, float *out) {
+__device__ __forceinline__ uint32_t contribution(const FactorArgs &a, const Staging<N> &st, uint32_t p, uint32_t cnt,
+                                                 uint32_t col, float *out) {
 #pragma unroll
     for (int k = 0; k < V; ++k) out[k] = 0.f;
     if (p >= cnt) return INVALID_ROW;
-    const uint32_t li = perm ? (uint32_t)perm[p] : p;
+    const uint32_t li = p;
     const uint32_t row = st.coord[a.mode][li];
     if (col < a.R) {
         const float v = st.val[li];
@@ -225,13 +237,11 @@ __device__ __forceinline__ void flush(const FactorArgs &a, uint32_t row, uint32_
     }
 }
 
-// Walk a staged chunk in sorted order, reduce runs of equal output rows and
-// add them to the output (see the header comment, steps 3-4).
+// Walk a staged chunk (sorted by output row), reduce runs of equal output
+// rows and add them to the output (the COO scheme of the header comment).
 template <int N, int V, int G, typename FT>
-__device__ __forceinline__ void process_chunk(const FactorArgs &a, const Staging<N> &st, const uint8_t *perm,
-                                              uint32_t cnt, int lane) {
+__device__ __forceinline__ void process_chunk(const FactorArgs &a, const Staging<N> &st, uint32_t cnt, int lane) {
     constexpr int NG = 64 / G;
-    constexpr int STEPS = CHUNK / NG;
     const int g = lane / G;
     const int s = lane % G;
     const uint32_t col = (uint32_t)s * V;
@@ -242,7 +252,7 @@ __device__ __forceinline__ void process_chunk(const FactorArgs &a, const Staging
     uint32_t carry_row = INVALID_ROW;
 
     float nxt[V];
-    uint32_t nxt_row = contribution<N, V, FT>(a, st, perm, (uint32_t)g, cnt, col, nxt);
+    uint32_t nxt_row = contribution<N, V, FT>(a, st, (uint32_t)g, cnt, col, nxt);
     const int nsteps = (int)((cnt + NG - 1) / NG);
     for (int step = 0; step < nsteps; ++step) {
         float val[V];
@@ -250,7 +260,7 @@ __device__ __forceinline__ void process_chunk(const FactorArgs &a, const Staging
         for (int k = 0; k < V; ++k) val[k] = nxt[k];
         uint32_t row = nxt_row;
         if (step + 1 < nsteps)
-            nxt_row = contribution<N, V, FT>(a, st, perm, (uint32_t)((step + 1) * NG + g), cnt, col, nxt);
+            nxt_row = contribution<N, V, FT>(a, st, (uint32_t)((step + 1) * NG + g), cnt, col, nxt);
 
         // Group 0 continues or closes the segment carried from the last step.
         if (g == 0 && carry_row != INVALID_ROW) {
@@ -290,16 +300,61 @@ __device__ __forceinline__ void process_chunk(const FactorArgs &a, const Staging
     if (g == 0 && carry_row != INVALID_ROW) flush<V>(a, carry_row, col, carry);
 }
 
+// Add a staged chunk's contributions into the LDS tile (row pitch TP floats).
 template <int N, int V, int G, typename FT>
-__global__ void __launch_bounds__(256) hicoo_mttkrp_kernel(HiCOOArgs h, FactorArgs a) {
+__device__ __forceinline__ void accumulate_chunk(const FactorArgs &a, const Staging<N> &st, uint32_t cnt,
+                                                 uint32_t row_mask, float *tile, int lane) {
+    constexpr int NG = 64 / G;
+    constexpr int TP = G * V + 1;
+    const int g = lane / G;
+    const uint32_t col = (uint32_t)(lane % G) * V;
+    float nxt[V];
+    uint32_t nxt_row = contribution<N, V, FT>(a, st, (uint32_t)g, cnt, col, nxt);
+    for (uint32_t p = g; p < cnt; p += NG) {
+        float val[V];
+#pragma unroll
+        for (int k = 0; k < V; ++k) val[k] = nxt[k];
+        const uint32_t row = nxt_row & row_mask;
+        if (p + NG < cnt) nxt_row = contribution<N, V, FT>(a, st, p + NG, cnt, col, nxt);
+        if (col < a.R) {
+            float *dst = tile + row * TP + col;
+#pragma unroll
+            for (int k = 0; k < V; ++k) atomicAdd(dst + k, val[k]);
+        }
+    }
+}
+
+template <int N, int V, int G, typename FT>
+__global__ void __launch_bounds__(256) hicoo_tile_kernel(HiCOOArgs h, FactorArgs a) {
+    extern __shared__ float tile[];  // (1 << sb_bits) rows x TP floats
     __shared__ Staging<N> st_all[WAVES_PER_BLOCK];
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t chunk = remap_block(blockIdx.x, gridDim.x) * WAVES_PER_BLOCK + wid;
-    if (chunk >= h.nchunks) return;
-    Staging<N> &st = st_all[wid];
-    const uint32_t cnt = stage_hicoo<N>(h, chunk, st, lane);
-    const uint8_t *perm = h.perm ? h.perm + (uint64_t)chunk * CHUNK : nullptr;
-    process_chunk<N, V, G, FT>(a, st, perm, cnt, lane);
+    constexpr int GV = G * V;
+    constexpr int TP = GV + 1;
+    const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+    const uint32_t u = remap_block(blockIdx.x, gridDim.x);   // gridDim.x == nunits
+    const uint32_t rows = 1u << h.sb_bits;
+    for (uint32_t i = tid; i < rows * TP; i += 256) tile[i] = 0.f;
+    __syncthreads();
+    const uint32_t c0 = h.unit_chunk[u], c1 = h.unit_chunk[u + 1];
+    for (uint32_t c = c0 + wid; c < c1; c += WAVES_PER_BLOCK) {
+        const uint32_t cnt = stage_hicoo<N>(h, c, st_all[wid], lane);
+        accumulate_chunk<N, V, G, FT>(a, st_all[wid], cnt, rows - 1, tile, lane);
+    }
+    __syncthreads();
+    const uint32_t base = h.unit_br[u] << h.sb_bits;
+    const bool excl = h.unit_excl[u] != 0;
+    for (uint32_t i = tid; i < rows * GV; i += 256) {
+        const uint32_t r = i / GV, c = i % GV;
+        const uint32_t orow = base + r;
+        if (c >= a.R || orow >= h.nrows) continue;
+        const float v = tile[r * TP + c];
+        if (v == 0.f) continue;
+        float *dst = a.out + (uint64_t)orow * a.out_stride + c;
+        if (excl)
+            *dst = v;
+        else
+            unsafeAtomicAdd(dst, v);
+    }
 }
 
 template <int N, int V, int G, typename FT>
@@ -310,42 +365,7 @@ __global__ void __launch_bounds__(256) coo_mttkrp_kernel(COOArgs c, FactorArgs a
     if (chunk >= c.nchunks) return;
     Staging<N> &st = st_all[wid];
     const uint32_t cnt = stage_coo<N>(c, chunk, st, lane);
-    process_chunk<N, V, G, FT>(a, st, nullptr, cnt, lane);
-}
-
-// Builds perm for one mode: each 256-thread workgroup sorts one chunk's
-// (row, local index) keys with a bitonic network in LDS.
-template <int N>
-__global__ void __launch_bounds__(256) hicoo_sort_chunks_kernel(HiCOOArgs h, uint32_t mode, uint8_t *perm) {
-    __shared__ Staging<N> st;
-    __shared__ uint64_t key[CHUNK];
-    const uint32_t chunk = blockIdx.x;
-    const int t = threadIdx.x;
-    uint32_t cnt = 0;
-    if (t < 64) cnt = stage_hicoo<N>(h, chunk, st, t);
-    __syncthreads();
-    {
-        const uint64_t z0 = (uint64_t)chunk * CHUNK;
-        const uint64_t rem = h.nnz - z0;
-        cnt = rem < CHUNK ? (uint32_t)rem : (uint32_t)CHUNK;
-    }
-    key[t] = (uint32_t)t < cnt ? (((uint64_t)st.coord[mode][t] << 8) | (uint64_t)t) : ~0ull;
-    __syncthreads();
-    for (int k = 2; k <= CHUNK; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            int ixj = t ^ j;
-            if (ixj > t) {
-                uint64_t x = key[t], y = key[ixj];
-                bool up = (t & k) == 0;
-                if ((x > y) == up) {
-                    key[t] = y;
-                    key[ixj] = x;
-                }
-            }
-            __syncthreads();
-        }
-    }
-    if ((uint32_t)t < cnt) perm[(uint64_t)chunk * CHUNK + t] = (uint8_t)(key[t] & 0xFFu);
+    process_chunk<N, V, G, FT>(a, st, cnt, lane);
 }
 
 }  // namespace dev
