@@ -2,23 +2,27 @@
 //
 // Device layout.  For every target mode m the upload keeps one copy of the
 // HiCOO blocks, stably reordered so that blocks with the same mode-m block
-// coordinate (a "block-row": 2^sb_bits output rows) are contiguous.  Blocks,
-// their 8-bit element offsets and values are the host tensor's, unchanged;
-// only the block order differs per mode.  Per mode:
-//   bptr       u32[nb + 1]       nonzero offsets of the reordered blocks
-//   binds      u32[nb * N]       block coordinates, block-major
+// coordinate (a "block-row": 2^sb_bits output rows) are contiguous.  Block
+// coordinates, 8-bit element offsets and values are the host tensor's.  The
+// nonzeros of each block-row are cut into chunks of <= 256; inside a chunk
+// they are ordered (stably) by their mode-m element offset, i.e. by output
+// row, and each carries its block's index relative to the chunk's first
+// block.  Per mode:
+//   binds      u32[nb * N]       block coordinates, block-major, reordered
 //   einds      u32[nnz * EW]     the N element offsets of a nonzero packed
 //                                into EW = ceil(N/4) words
 //   vals       f32[nnz]
-//   chunk_z    u32[nchunks + 1]  chunks of <= 256 nonzeros that never cross
-//   chunk_blk  u32[nchunks + 1]  a block-row; first nonzero and its block
+//   lblk       u8[nnz]           block index - chunk_blk[chunk]  (< 256)
+//   chunk_z    u32[nchunks + 1]  first nonzero of each chunk
+//   chunk_blk  u32[nchunks]      first block of each chunk
 //   unit_chunk u32[nunits + 1]   units: runs of <= UNIT_CHUNKS chunks of one
 //   unit_br    u32[nunits]       block-row, each processed by one workgroup
 //   unit_excl  u32[nunits]       1 if the block-row has a single unit
-// N copies cost N times the memory of one (6.8 GB for a 100M-nonzero order-3
-// tensor with 73M blocks), which HBM holds easily; in exchange each MTTKRP
-// streams its copy sequentially and accumulates outputs in LDS.
-// The host HiCOO struct is not referenced after the upload.
+// N copies cost N times the memory of one (about 6 GB for a 100M-nonzero
+// order-3 tensor with 73M blocks), which HBM holds easily; in exchange each
+// MTTKRP streams its copy sequentially, reduces equal rows in registers and
+// accumulates outputs in LDS.  The host HiCOO struct is not referenced after
+// the upload.
 #include <atomic>
 #include <mutex>
 #include <unordered_map>
@@ -33,7 +37,7 @@ constexpr uint32_t UNIT_CHUNKS = 64;  // chunks (of 256 nonzeros) per workgroup 
 }
 
 struct ptiDevHiCOOMode {
-    pti::dev::Buffer bptr, binds, einds, vals, chunk_z, chunk_blk, unit_chunk, unit_br, unit_excl;
+    pti::dev::Buffer binds, einds, vals, lblk, chunk_z, chunk_blk, unit_chunk, unit_br, unit_excl;
     uint32_t nchunks = 0, nunits = 0;
     bool ready = false;
 };
@@ -54,16 +58,15 @@ namespace dev {
 HiCOOArgs hicoo_args(const ptiDevHiCOO *t, uint32_t mode) {
     const ptiDevHiCOOMode &c = t->mc[mode];
     HiCOOArgs h;
-    h.bptr = c.bptr.as<uint32_t>();
     h.binds = c.binds.as<uint32_t>();
     h.einds = c.einds.as<uint32_t>();
     h.vals = c.vals.as<float>();
+    h.lblk = c.lblk.as<uint8_t>();
     h.chunk_z = c.chunk_z.as<uint32_t>();
     h.chunk_blk = c.chunk_blk.as<uint32_t>();
     h.unit_chunk = c.unit_chunk.as<uint32_t>();
     h.unit_br = c.unit_br.as<uint32_t>();
     h.unit_excl = c.unit_excl.as<uint32_t>();
-    h.nb = (uint32_t)t->nb;
     h.nunits = c.nunits;
     h.sb_bits = t->sb_bits;
     h.nrows = t->ndims[mode];
@@ -130,27 +133,16 @@ int build_mode(ptiDevHiCOO *t, const ptiSparseTensorHiCOO *hitsr, uint32_t mode,
     while (kbits < 32 && (kmax >> kbits)) ++kbits;
     pti::radix_sort_pairs<uint32_t>(key, ids, kbits, 0);
 
-    // Reordered blocks.
+    // Reordered blocks: nonzero offsets (block order) and coordinates.
     std::vector<uint32_t> hb(nb + 1), hbi(nb * N);
     hb[0] = 0;
     for (uint64_t i = 0; i < nb; ++i) {
         const uint64_t b = ids[i];
         hb[i + 1] = hb[i] + (uint32_t)(hitsr->bptr.data[b + 1] - hitsr->bptr.data[b]);
     }
-    std::vector<uint32_t> he(nnz * EW, 0u);
-    std::vector<float> hv(nnz);
     pti::parallel_for(nb, 0, [&](uint64_t lo, uint64_t hi, int) {
-        for (uint64_t i = lo; i < hi; ++i) {
-            const uint64_t b = ids[i];
-            for (uint32_t m = 0; m < N; ++m) hbi[i * N + m] = hitsr->binds[m].data[b];
-            const uint64_t src = hitsr->bptr.data[b], len = hitsr->bptr.data[b + 1] - src;
-            for (uint64_t j = 0; j < len; ++j) {
-                const uint64_t z = hb[i] + j;
-                for (uint32_t m = 0; m < N; ++m)
-                    he[z * EW + (m >> 2)] |= (uint32_t)hitsr->einds[m].data[src + j] << ((m & 3) * 8);
-                hv[z] = hitsr->values.data[src + j];
-            }
-        }
+        for (uint64_t i = lo; i < hi; ++i)
+            for (uint32_t m = 0; m < N; ++m) hbi[i * N + m] = hitsr->binds[m].data[ids[i]];
     });
 
     // Chunks (never crossing a block-row) and units.
@@ -181,8 +173,38 @@ int build_mode(ptiDevHiCOO *t, const ptiSparseTensorHiCOO *hitsr, uint32_t mode,
     c.nchunks = (uint32_t)cz.size();
     c.nunits = (uint32_t)ubr.size();
     cz.push_back((uint32_t)nnz);
-    cb.push_back((uint32_t)nb);
     uc.push_back(c.nchunks);
+
+    // Nonzeros: per chunk, a stable counting sort on the mode-m element
+    // offset; each nonzero records its block relative to the chunk's first.
+    std::vector<uint32_t> he(nnz * EW, 0u);
+    std::vector<float> hv(nnz);
+    std::vector<uint8_t> hl(nnz);
+    const uint32_t nrow = 1u << t->sb_bits;
+    pti::parallel_for(c.nchunks, 0, [&](uint64_t lo, uint64_t hi, int) {
+        std::vector<uint32_t> src(pti::dev::CHUNK), bl(pti::dev::CHUNK), cnt(nrow + 1);
+        for (uint64_t ch = lo; ch < hi; ++ch) {
+            const uint32_t z0 = cz[ch], n = cz[ch + 1] - z0;
+            // Source nonzero and block (in block order) of each position.
+            uint64_t blk = cb[ch];
+            for (uint32_t k = 0; k < n; ++k) {
+                while (hb[blk + 1] <= z0 + k) ++blk;
+                src[k] = (uint32_t)(hitsr->bptr.data[ids[blk]] + (z0 + k - hb[blk]));
+                bl[k] = (uint32_t)(blk - cb[ch]);
+            }
+            std::fill(cnt.begin(), cnt.end(), 0u);
+            for (uint32_t k = 0; k < n; ++k) ++cnt[hitsr->einds[mode].data[src[k]] + 1];
+            for (uint32_t r = 0; r < nrow; ++r) cnt[r + 1] += cnt[r];
+            for (uint32_t k = 0; k < n; ++k) {
+                const uint32_t sz = src[k];
+                const uint64_t z = z0 + cnt[hitsr->einds[mode].data[sz]]++;
+                for (uint32_t m = 0; m < N; ++m)
+                    he[z * EW + (m >> 2)] |= (uint32_t)hitsr->einds[m].data[sz] << ((m & 3) * 8);
+                hv[z] = hitsr->values.data[sz];
+                hl[z] = (uint8_t)bl[k];
+            }
+        }
+    });
 
     auto put = [&](Buffer &buf, const void *src, size_t bytes) -> int {
         PTI_HIP(buf.alloc(bytes), M);
@@ -191,10 +213,10 @@ int build_mode(ptiDevHiCOO *t, const ptiSparseTensorHiCOO *hitsr, uint32_t mode,
         return 0;
     };
     int rc = 0;
-    if (!rc) rc = put(c.bptr, hb.data(), hb.size() * 4);
     if (!rc) rc = put(c.binds, hbi.data(), hbi.size() * 4);
     if (!rc) rc = put(c.einds, he.data(), he.size() * 4);
     if (!rc) rc = put(c.vals, hv.data(), hv.size() * 4);
+    if (!rc) rc = put(c.lblk, hl.data(), hl.size());
     if (!rc) rc = put(c.chunk_z, cz.data(), cz.size() * 4);
     if (!rc) rc = put(c.chunk_blk, cb.data(), cb.size() * 4);
     if (!rc) rc = put(c.unit_chunk, uc.data(), uc.size() * 4);

@@ -10,23 +10,28 @@
 //
 // HiCOO kernel (hicoo_tile_kernel).  The device keeps, for each target mode
 // m, a copy of the HiCOO blocks grouped by their mode-m block coordinate
-// (block-row); blocks, 8-bit element offsets and values are unchanged.  A
-// workgroup owns one "unit": a run of chunks inside one block-row.  All
-// nonzeros of a unit write output rows (block-row << sb_bits) + eind_m, i.e.
-// 2^sb_bits rows, which the workgroup accumulates in an LDS tile indexed by
-// the 8-bit element offset.  At the end the tile is written once: with plain
-// stores when the unit is the block-row's only unit, otherwise with one FP32
-// atomic per element.  Global atomics thus scale with the number of units,
-// not with the number of nonzeros (a Zipf head row no longer serialises
-// millions of atomics).
+// (block-row); block coordinates, 8-bit element offsets and values are the
+// tensor's.  The copy is cut into chunks of <= 256 nonzeros that never cross
+// a block-row; inside a chunk the nonzeros are ordered by their mode-m
+// element offset (i.e. by output row) and each carries the index of its
+// block relative to the chunk's first block (1 byte).  A workgroup owns one
+// "unit": a run of chunks of one block-row.  Its waves reduce runs of equal
+// output rows in registers (segmented scan below) and add each run once into
+// an LDS tile of the block-row's 2^sb_bits output rows, indexed by the 8-bit
+// element offset.  At the end the tile is written once: with plain stores
+// when the unit is the block-row's only unit, otherwise with one FP32 atomic
+// per element.  Global atomics thus scale with the number of units, not with
+// the number of nonzeros, and a Zipf head row no longer serialises millions
+// of atomics.
 //
 // COO kernel (coo_mttkrp_kernel).  Per target mode the device keeps a copy
-// sorted by the output index.  Within a step the NG contributions are
+// sorted by the output index; each run of equal rows in a chunk is added to
+// the output with one FP32 atomic per rank.
+//
+// Run reduction (both kernels).  Within a step the NG contributions are
 // combined by a segmented inclusive scan over groups keyed on the output row;
-// the last group of a segment adds the segment sum to the output with one
-// FP32 atomic per rank, and a segment still open at the end of a step is
-// carried into the next step, so each output row receives one atomic per
-// (chunk, row) run.
+// the last group of a segment passes the segment sum to the sink, and a
+// segment still open at the end of a step is carried into the next step.
 //
 // Workgroup ids are remapped so that consecutive work lands on the same XCD
 // (the hardware deals workgroups to the 8 XCDs round-robin), which keeps the
@@ -56,16 +61,15 @@ struct FactorArgs {
 
 // One target mode's copy of a HiCOO tensor (see the header comment).
 struct HiCOOArgs {
-    const uint32_t *bptr;        // nb + 1 nonzero offsets (32-bit)
     const uint32_t *binds;       // nb * N block coordinates, block-major
     const uint32_t *einds;       // nnz * EW packed 8-bit element offsets
     const float *vals;           // nnz values
+    const uint8_t *lblk;         // nnz: block of the nonzero minus its chunk's first block
     const uint32_t *chunk_z;     // nchunks + 1: first nonzero of each chunk; [nchunks] = nnz
-    const uint32_t *chunk_blk;   // nchunks + 1: block holding chunk_z[c]; [nchunks] = nb
+    const uint32_t *chunk_blk;   // nchunks: first block of each chunk
     const uint32_t *unit_chunk;  // nunits + 1: first chunk of each unit
     const uint32_t *unit_br;     // nunits: block-row (mode-m block coordinate) of each unit
     const uint32_t *unit_excl;   // nunits: 1 if the unit is its block-row's only unit
-    uint32_t nb;
     uint32_t nunits;
     uint32_t sb_bits;
     uint32_t nrows;              // ndims of the target mode
@@ -139,7 +143,6 @@ template <int N>
 struct Staging {
     uint32_t coord[N][CHUNK];
     float val[CHUNK];
-    uint32_t bptr[CHUNK + 4];
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -148,38 +151,26 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Decode chunk `chunk` of a HiCOO mode copy into st (one wave, 64 lanes).
-// A chunk holds at most CHUNK nonzeros, all blocks non-empty, so it spans at
-// most CHUNK + 1 blocks and bptr[b0 .. blast] fits in st.bptr.
-// Returns the number of nonzeros in the chunk.
+// Decode chunk `chunk` of a HiCOO mode copy into st (one wave, 64 lanes):
+// coordinate = (binds[first block + lblk] << sb_bits) + element offset.
+// Returns the number of nonzeros in the chunk (<= CHUNK).
 template <int N>
 __device__ __forceinline__ uint32_t stage_hicoo(const HiCOOArgs &h, uint32_t chunk, Staging<N> &st, int lane) {
     constexpr int EW = (N + 3) / 4;
     const uint32_t z0 = h.chunk_z[chunk];
     const uint32_t cnt = h.chunk_z[chunk + 1] - z0;
     const uint32_t b0 = h.chunk_blk[chunk];
-    const uint32_t b1 = h.chunk_blk[chunk + 1];
-    const uint32_t blast = (b1 + 1 < h.nb) ? b1 + 1 : h.nb;   // inclusive bptr index
-    const uint32_t nbp = blast - b0 + 1;                      // <= CHUNK + 2
-    wave_sync();
-    for (uint32_t j = lane; j < nbp; j += 64) st.bptr[j] = h.bptr[b0 + j];
     wave_sync();
     for (uint32_t zl = lane; zl < cnt; zl += 64) {
         const uint32_t z = z0 + zl;
-        // Largest j with bptr[b0 + j] <= z, j in [0, nbp - 2].
-        uint32_t lo = 0, hi = nbp - 2;
-        while (lo < hi) {
-            uint32_t mid = (lo + hi + 1) >> 1;
-            if (st.bptr[mid] <= z) lo = mid; else hi = mid - 1;
-        }
-        const uint32_t b = b0 + lo;
+        const uint64_t b = (uint64_t)b0 + h.lblk[z];
         uint32_t ew[EW];
 #pragma unroll
         for (int w = 0; w < EW; ++w) ew[w] = h.einds[(uint64_t)z * EW + w];
 #pragma unroll
         for (int m = 0; m < N; ++m) {
-            uint32_t e = (ew[m >> 2] >> ((m & 3) * 8)) & 0xFFu;
-            st.coord[m][zl] = (h.binds[(uint64_t)b * N + m] << h.sb_bits) + e;
+            const uint32_t e = (ew[m >> 2] >> ((m & 3) * 8)) & 0xFFu;
+            st.coord[m][zl] = (h.binds[b * N + m] << h.sb_bits) + e;
         }
         st.val[zl] = h.vals[z];
     }
@@ -228,19 +219,37 @@ __device__ __forceinline__ uint32_t contribution(const FactorArgs &a, const Stag
     return row;
 }
 
-template <int V>
-__device__ __forceinline__ void flush(const FactorArgs &a, uint32_t row, uint32_t col, const float *val) {
-    if (col < a.R) {
-        float *dst = a.out + (uint64_t)row * a.out_stride + col;
+// Where a reduced run of one output row goes.
+// GlobalSink: FP32 atomics into the output matrix (COO path).
+struct GlobalSink {
+    float *out;
+    uint32_t out_stride;
+    template <int V>
+    __device__ __forceinline__ void add(uint32_t row, uint32_t col, const float *val) const {
+        float *dst = out + (uint64_t)row * out_stride + col;
 #pragma unroll
         for (int k = 0; k < V; ++k) unsafeAtomicAdd(dst + k, val[k]);
     }
-}
+};
+// TileSink: LDS atomics into the workgroup's output tile (HiCOO path); the
+// tile row is the row's offset inside its block-row.
+template <int TP>
+struct TileSink {
+    float *tile;
+    uint32_t mask;
+    template <int V>
+    __device__ __forceinline__ void add(uint32_t row, uint32_t col, const float *val) const {
+        float *dst = tile + (row & mask) * TP + col;
+#pragma unroll
+        for (int k = 0; k < V; ++k) atomicAdd(dst + k, val[k]);
+    }
+};
 
 // Walk a staged chunk (sorted by output row), reduce runs of equal output
-// rows and add them to the output (the COO scheme of the header comment).
-template <int N, int V, int G, typename FT>
-__device__ __forceinline__ void process_chunk(const FactorArgs &a, const Staging<N> &st, uint32_t cnt, int lane) {
+// rows across the lane groups and hand each run's sum to the sink once.
+template <int N, int V, int G, typename FT, typename Sink>
+__device__ __forceinline__ void process_chunk(const FactorArgs &a, const Staging<N> &st, uint32_t cnt,
+                                              const Sink &sink, int lane) {
     constexpr int NG = 64 / G;
     const int g = lane / G;
     const int s = lane % G;
@@ -268,7 +277,7 @@ __device__ __forceinline__ void process_chunk(const FactorArgs &a, const Staging
 #pragma unroll
                 for (int k = 0; k < V; ++k) val[k] += carry[k];
             } else {
-                flush<V>(a, carry_row, col, carry);
+                if (col < a.R) sink.template add<V>(carry_row, col, carry);
             }
         }
         // Segment heads: first group, or row differs from the previous group.
@@ -290,38 +299,14 @@ __device__ __forceinline__ void process_chunk(const FactorArgs &a, const Staging
         }
         uint32_t next_row = __shfl_down(row, G, 64);
         const bool last_group = (g == NG - 1);
-        if (!last_group && next_row != row && row != INVALID_ROW) flush<V>(a, row, col, val);
+        if (!last_group && next_row != row && row != INVALID_ROW && col < a.R) sink.template add<V>(row, col, val);
         // The last group's segment stays open: hand it to group 0.
         const int src = (NG - 1) * G + s;
         carry_row = __shfl(row, (NG - 1) * G, 64);
 #pragma unroll
         for (int k = 0; k < V; ++k) carry[k] = __shfl(val[k], src, 64);
     }
-    if (g == 0 && carry_row != INVALID_ROW) flush<V>(a, carry_row, col, carry);
-}
-
-// Add a staged chunk's contributions into the LDS tile (row pitch TP floats).
-template <int N, int V, int G, typename FT>
-__device__ __forceinline__ void accumulate_chunk(const FactorArgs &a, const Staging<N> &st, uint32_t cnt,
-                                                 uint32_t row_mask, float *tile, int lane) {
-    constexpr int NG = 64 / G;
-    constexpr int TP = G * V + 1;
-    const int g = lane / G;
-    const uint32_t col = (uint32_t)(lane % G) * V;
-    float nxt[V];
-    uint32_t nxt_row = contribution<N, V, FT>(a, st, (uint32_t)g, cnt, col, nxt);
-    for (uint32_t p = g; p < cnt; p += NG) {
-        float val[V];
-#pragma unroll
-        for (int k = 0; k < V; ++k) val[k] = nxt[k];
-        const uint32_t row = nxt_row & row_mask;
-        if (p + NG < cnt) nxt_row = contribution<N, V, FT>(a, st, p + NG, cnt, col, nxt);
-        if (col < a.R) {
-            float *dst = tile + row * TP + col;
-#pragma unroll
-            for (int k = 0; k < V; ++k) atomicAdd(dst + k, val[k]);
-        }
-    }
+    if (g == 0 && carry_row != INVALID_ROW && col < a.R) sink.template add<V>(carry_row, col, carry);
 }
 
 template <int N, int V, int G, typename FT>
@@ -336,9 +321,10 @@ __global__ void __launch_bounds__(256) hicoo_tile_kernel(HiCOOArgs h, FactorArgs
     for (uint32_t i = tid; i < rows * TP; i += 256) tile[i] = 0.f;
     __syncthreads();
     const uint32_t c0 = h.unit_chunk[u], c1 = h.unit_chunk[u + 1];
+    const TileSink<TP> sink{tile, rows - 1};
     for (uint32_t c = c0 + wid; c < c1; c += WAVES_PER_BLOCK) {
         const uint32_t cnt = stage_hicoo<N>(h, c, st_all[wid], lane);
-        accumulate_chunk<N, V, G, FT>(a, st_all[wid], cnt, rows - 1, tile, lane);
+        process_chunk<N, V, G, FT>(a, st_all[wid], cnt, sink, lane);
     }
     __syncthreads();
     const uint32_t base = h.unit_br[u] << h.sb_bits;
@@ -365,7 +351,7 @@ __global__ void __launch_bounds__(256) coo_mttkrp_kernel(COOArgs c, FactorArgs a
     if (chunk >= c.nchunks) return;
     Staging<N> &st = st_all[wid];
     const uint32_t cnt = stage_coo<N>(c, chunk, st, lane);
-    process_chunk<N, V, G, FT>(a, st, cnt, lane);
+    process_chunk<N, V, G, FT>(a, st, cnt, GlobalSink{a.out, a.out_stride}, lane);
 }
 
 }  // namespace dev
