@@ -15,10 +15,11 @@
 // a block-row; inside a chunk the nonzeros are ordered by their mode-m
 // element offset (i.e. by output row) and each carries the index of its
 // block relative to the chunk's first block (1 byte).  A workgroup owns one
-// "unit": a run of chunks of one block-row.  Its waves reduce runs of equal
-// output rows in registers (segmented scan below) and add each run once into
-// an LDS tile of the block-row's 2^sb_bits output rows, indexed by the 8-bit
-// element offset.  At the end the tile is written once: with plain stores
+// "unit": a run of chunks of one block-row.  Its waves add into an LDS tile
+// of the block-row's 2^sb_bits output rows, indexed by the 8-bit element
+// offset.  Each lane group walks a contiguous slice of the chunk and
+// sums runs of equal rows in registers, so a run reaches the tile once per
+// slice.  At the end the tile is written once: with plain stores
 // when the unit is the block-row's only unit, otherwise with one FP32 atomic
 // per element.  Global atomics thus scale with the number of units, not with
 // the number of nonzeros, and a Zipf head row no longer serialises millions
@@ -28,7 +29,7 @@
 // sorted by the output index; each run of equal rows in a chunk is added to
 // the output with one FP32 atomic per rank.
 //
-// Run reduction (both kernels).  Within a step the NG contributions are
+// Run reduction (COO kernel).  Within a step the NG contributions are
 // combined by a segmented inclusive scan over groups keyed on the output row;
 // the last group of a segment passes the segment sum to the sink, and a
 // segment still open at the end of a step is carried into the next step.
@@ -139,10 +140,15 @@ struct FactorLoad<uint16_t, 1> {
     }
 };
 
+// Staged chunk.  Position p is stored at p + p/32, so that the lane groups
+// of a wave reading positions 32 apart hit different LDS banks.
+__device__ __forceinline__ uint32_t skew(uint32_t p) { return p + (p >> 5); }
+constexpr int CHUNK_SKEWED = CHUNK + CHUNK / 32;
+
 template <int N>
 struct Staging {
-    uint32_t coord[N][CHUNK];
-    float val[CHUNK];
+    uint32_t coord[N][CHUNK_SKEWED];
+    float val[CHUNK_SKEWED];
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -170,9 +176,9 @@ __device__ __forceinline__ uint32_t stage_hicoo(const HiCOOArgs &h, uint32_t chu
 #pragma unroll
         for (int m = 0; m < N; ++m) {
             const uint32_t e = (ew[m >> 2] >> ((m & 3) * 8)) & 0xFFu;
-            st.coord[m][zl] = (h.binds[b * N + m] << h.sb_bits) + e;
+            st.coord[m][skew(zl)] = (h.binds[b * N + m] << h.sb_bits) + e;
         }
-        st.val[zl] = h.vals[z];
+        st.val[skew(zl)] = h.vals[z];
     }
     wave_sync();
     return cnt;
@@ -186,8 +192,8 @@ __device__ __forceinline__ uint32_t stage_coo(const COOArgs &c, uint32_t chunk, 
     wave_sync();
     for (uint32_t zl = lane; zl < cnt; zl += 64) {
 #pragma unroll
-        for (int m = 0; m < N; ++m) st.coord[m][zl] = c.inds[m][z0 + zl];
-        st.val[zl] = c.vals[z0 + zl];
+        for (int m = 0; m < N; ++m) st.coord[m][skew(zl)] = c.inds[m][z0 + zl];
+        st.val[skew(zl)] = c.vals[z0 + zl];
     }
     wave_sync();
     return cnt;
@@ -201,7 +207,7 @@ __device__ __forceinline__ uint32_t contribution(const FactorArgs &a, const Stag
 #pragma unroll
     for (int k = 0; k < V; ++k) out[k] = 0.f;
     if (p >= cnt) return INVALID_ROW;
-    const uint32_t li = p;
+    const uint32_t li = skew(p);
     const uint32_t row = st.coord[a.mode][li];
     if (col < a.R) {
         const float v = st.val[li];
@@ -309,6 +315,47 @@ __device__ __forceinline__ void process_chunk(const FactorArgs &a, const Staging
     if (g == 0 && carry_row != INVALID_ROW && col < a.R) sink.template add<V>(carry_row, col, carry);
 }
 
+// Walk a staged chunk (sorted by output row) with each lane group taking a
+// contiguous slice of positions: equal rows are summed in registers and each
+// run is handed to the sink once.  No cross-lane traffic is needed; a row
+// that spans several slices is added once per slice.
+template <int N, int V, int G, typename FT, typename Sink>
+__device__ __forceinline__ void process_slices(const FactorArgs &a, const Staging<N> &st, uint32_t cnt,
+                                               const Sink &sink, int lane) {
+    constexpr int NG = 64 / G;
+    const uint32_t g = (uint32_t)lane / G;
+    const uint32_t col = (uint32_t)(lane % G) * V;
+    const uint32_t per = (cnt + NG - 1) / NG;
+    const uint32_t p0 = g * per;
+    const uint32_t p1 = p0 + per < cnt ? p0 + per : cnt;
+    float acc[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) acc[k] = 0.f;
+    uint32_t cur = INVALID_ROW;
+    float nxt[V];
+    uint32_t nxt_row = contribution<N, V, FT>(a, st, p0, p1, col, nxt);
+    for (uint32_t i = 0; i < per; ++i) {
+        const uint32_t p = p0 + i;
+        float val[V];
+#pragma unroll
+        for (int k = 0; k < V; ++k) val[k] = nxt[k];
+        const uint32_t row = nxt_row;
+        if (i + 1 < per) nxt_row = contribution<N, V, FT>(a, st, p + 1, p1, col, nxt);
+        if (p < p1) {
+            if (row != cur) {
+                if (cur != INVALID_ROW && col < a.R) sink.template add<V>(cur, col, acc);
+#pragma unroll
+                for (int k = 0; k < V; ++k) acc[k] = val[k];
+                cur = row;
+            } else {
+#pragma unroll
+                for (int k = 0; k < V; ++k) acc[k] += val[k];
+            }
+        }
+    }
+    if (cur != INVALID_ROW && col < a.R) sink.template add<V>(cur, col, acc);
+}
+
 template <int N, int V, int G, typename FT>
 __global__ void __launch_bounds__(256) hicoo_tile_kernel(HiCOOArgs h, FactorArgs a) {
     extern __shared__ float tile[];  // (1 << sb_bits) rows x TP floats
@@ -324,7 +371,7 @@ __global__ void __launch_bounds__(256) hicoo_tile_kernel(HiCOOArgs h, FactorArgs
     const TileSink<TP> sink{tile, rows - 1};
     for (uint32_t c = c0 + wid; c < c1; c += WAVES_PER_BLOCK) {
         const uint32_t cnt = stage_hicoo<N>(h, c, st_all[wid], lane);
-        process_chunk<N, V, G, FT>(a, st_all[wid], cnt, sink, lane);
+        process_slices<N, V, G, FT>(a, st_all[wid], cnt, sink, lane);
     }
     __syncthreads();
     const uint32_t base = h.unit_br[u] << h.sb_bits;
