@@ -46,7 +46,22 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-bf16", action="store_true")
     p.add_argument("--cpu-sample-nnz", type=float, default=8e6)
+    p.add_argument("--copy-calibration", type=float, default=0.0,
+                   help="only copy a GiB-sized float buffer 3 times (HBM counter calibration) and exit")
     return p.parse_args()
+
+
+def copy_calibration(gib):
+    """Plain device-to-device copies of a known size, so that rocprofv3's
+    FETCH_SIZE / WRITE_SIZE readings can be scaled to bytes."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_copy": n * 4, "copies": 3}), flush=True)
 
 
 def log(*a):
@@ -95,6 +110,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     import torch
     from hiparti_amd import COOTensor, DenseMatrix, lib
     from hiparti_amd.device import DevHiCOO

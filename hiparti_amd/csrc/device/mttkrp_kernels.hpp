@@ -318,43 +318,85 @@ __device__ __forceinline__ void process_chunk(const FactorArgs &a, const Staging
 // Walk a staged chunk (sorted by output row) with each lane group taking a
 // contiguous slice of positions: equal rows are summed in registers and each
 // run is handed to the sink once.  No cross-lane traffic is needed; a row
-// that spans several slices is added once per slice.
-template <int N, int V, int G, typename FT, typename Sink>
+// that spans several slices is added once per slice.  The factor-row gathers
+// are software-pipelined D positions deep (slot j of the ring is consumed and
+// immediately refilled with the position D further on), so every lane keeps
+// D*(N-1) row loads in flight instead of waiting on each one.
+template <int N, int V, int G, int D, typename FT, typename Sink>
 __device__ __forceinline__ void process_slices(const FactorArgs &a, const Staging<N> &st, uint32_t cnt,
                                                const Sink &sink, int lane) {
     constexpr int NG = 64 / G;
     const uint32_t g = (uint32_t)lane / G;
     const uint32_t col = (uint32_t)(lane % G) * V;
+    const bool active = col < a.R;
     const uint32_t per = (cnt + NG - 1) / NG;
     const uint32_t p0 = g * per;
     const uint32_t p1 = p0 + per < cnt ? p0 + per : cnt;
+    float fa[D][N - 1][V];
+    float vv[D];
+    uint32_t rr[D];
+    auto issue = [&](int j, uint32_t p) {
+        if (p < p1) {
+            const uint32_t li = skew(p);
+            rr[j] = st.coord[a.mode][li];
+            vv[j] = st.val[li];
+            if (active) {
+#pragma unroll
+                for (int i = 1; i < N; ++i)
+                    FactorLoad<FT, V>::load(a.fac[a.order[i]], (uint64_t)st.coord[a.order[i]][li] * a.stride + col,
+                                            fa[j][i - 1]);
+            } else {
+#pragma unroll
+                for (int i = 1; i < N; ++i)
+#pragma unroll
+                    for (int k = 0; k < V; ++k) fa[j][i - 1][k] = 0.f;
+            }
+        } else {
+            rr[j] = INVALID_ROW;
+        }
+    };
+#pragma unroll
+    for (int j = 0; j < D; ++j) issue(j, p0 + j);
     float acc[V];
 #pragma unroll
     for (int k = 0; k < V; ++k) acc[k] = 0.f;
     uint32_t cur = INVALID_ROW;
-    float nxt[V];
-    uint32_t nxt_row = contribution<N, V, FT>(a, st, p0, p1, col, nxt);
-    for (uint32_t i = 0; i < per; ++i) {
-        const uint32_t p = p0 + i;
-        float val[V];
+    for (uint32_t base = 0; base < per; base += D) {
 #pragma unroll
-        for (int k = 0; k < V; ++k) val[k] = nxt[k];
-        const uint32_t row = nxt_row;
-        if (i + 1 < per) nxt_row = contribution<N, V, FT>(a, st, p + 1, p1, col, nxt);
-        if (p < p1) {
-            if (row != cur) {
-                if (cur != INVALID_ROW && col < a.R) sink.template add<V>(cur, col, acc);
+        for (int j = 0; j < D; ++j) {
+            const uint32_t row = rr[j];
+            float prod[V];
+            if (row != INVALID_ROW) {
 #pragma unroll
-                for (int k = 0; k < V; ++k) acc[k] = val[k];
-                cur = row;
-            } else {
+                for (int k = 0; k < V; ++k) prod[k] = vv[j] * fa[j][0][k];
 #pragma unroll
-                for (int k = 0; k < V; ++k) acc[k] += val[k];
+                for (int i = 1; i < N - 1; ++i)
+#pragma unroll
+                    for (int k = 0; k < V; ++k) prod[k] *= fa[j][i][k];
+            }
+            issue(j, p0 + base + j + D);
+            if (row != INVALID_ROW) {
+                if (row != cur) {
+                    if (cur != INVALID_ROW && active) sink.template add<V>(cur, col, acc);
+#pragma unroll
+                    for (int k = 0; k < V; ++k) acc[k] = prod[k];
+                    cur = row;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < V; ++k) acc[k] += prod[k];
+                }
             }
         }
     }
-    if (cur != INVALID_ROW && col < a.R) sink.template add<V>(cur, col, acc);
+    if (cur != INVALID_ROW && active) sink.template add<V>(cur, col, acc);
 }
+
+// Pipeline depth of the gathers: deeper for low orders, where the ring of
+// D * (N - 1) * V loaded values stays small.
+template <int N, int V>
+struct PipeDepth {
+    static constexpr int value = (N - 1) * V <= 8 ? 4 : ((N - 1) * V <= 16 ? 2 : 1);
+};
 
 template <int N, int V, int G, typename FT>
 __global__ void __launch_bounds__(256) hicoo_tile_kernel(HiCOOArgs h, FactorArgs a) {
@@ -371,7 +413,7 @@ __global__ void __launch_bounds__(256) hicoo_tile_kernel(HiCOOArgs h, FactorArgs
     const TileSink<TP> sink{tile, rows - 1};
     for (uint32_t c = c0 + wid; c < c1; c += WAVES_PER_BLOCK) {
         const uint32_t cnt = stage_hicoo<N>(h, c, st_all[wid], lane);
-        process_slices<N, V, G, FT>(a, st_all[wid], cnt, sink, lane);
+        process_slices<N, V, G, PipeDepth<N, V>::value, FT>(a, st_all[wid], cnt, sink, lane);
     }
     __syncthreads();
     const uint32_t base = h.unit_br[u] << h.sb_bits;
