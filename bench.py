@@ -1,4 +1,4 @@
-"""Benchmark: HiCOO MTTKRP on one MI355X (configs[1] workload).
+"""Benchmark: HiCOO MTTKRP on one MI355X.
 
 Workload: order-3 tensor 2^20 x 2^20 x 2^16, 100M unique nonzeros, each
 mode's index Zipf(1.1) over its dimension and randomly relabelled, values
@@ -107,8 +107,8 @@ def cpu_baseline(h, ndims, facs_np, R, sample_nnz, nthreads):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (MTTKRP would shard by output "
+              "block-row ownership); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)

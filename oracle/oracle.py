@@ -2,8 +2,8 @@
 
 numpy front-end to oracle/liboracle.so (pti_oracle.c), the CPU restatement of
 the reference's MTTKRP path.  Only tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py use it, and only as the checker or the reported
-CPU baseline.  Build: make -C oracle.
+CPU-baseline leg of bench.py use it, and only to check results or as the
+reported CPU baseline.  Build: make -C oracle.
 """
 import ctypes as C
 import os

@@ -2,7 +2,7 @@
  * ORACLE -- test infrastructure only.
  *
  * CPU restatement of the reference HiParTI algorithms on the MTTKRP path,
- * used as the checker for libHiParTI.so.  Only tests/, __graft_entry__.smoke()
+ * used to check libHiParTI.so's results.  Only tests/, __graft_entry__.smoke()
  * and bench.py's CPU-baseline leg may load it; the product never does.
  *
  * All arrays are plain C arrays so the functions can be driven from ctypes.

@@ -1,4 +1,4 @@
-"""Pins the oracle (oracle/pti_oracle.c) before it is trusted as the checker.
+"""Pins the oracle (oracle/pti_oracle.c) before its results are trusted.
 
 Against the reference's own outputs (tests/golden/*.npz, written by
 tests/golden/make_goldens.py from the compiled reference): the restated
