@@ -123,6 +123,8 @@ SIGNATURES = {
     "ptiDevCOOFree": (None, [_vp]),
     "ptiDevMTTKRPHiCOO": (C.c_int, [_vp, _P(_vp), ptiIndex, ptiIndex, _P(ptiIndex), ptiIndex, _vp]),
     "ptiDevMTTKRPCOO": (C.c_int, [_vp, _P(_vp), ptiIndex, ptiIndex, _P(ptiIndex), ptiIndex, _vp]),
+    "ptiDevMTTKRPCOOValues": (C.c_int, [_vp, _vp, _P(_vp), ptiIndex, ptiIndex, _P(ptiIndex), ptiIndex, _vp]),
+    "ptiDevCOOCompletionForward": (C.c_int, [_vp, _P(_vp), ptiIndex, ptiIndex, _vp, _vp, _vp]),
     "ptiDevMTTKRPHiCOO_BF16": (C.c_int, [_vp, _P(_vp), _vp, ptiIndex, ptiIndex, ptiIndex,
                                          _P(ptiIndex), ptiIndex, _vp]),
     "ptiRegisterHiCOO": (C.c_int, [_P(ptiSparseTensorHiCOO)]),

@@ -1,0 +1,123 @@
+"""CP-completion loss as a torch.autograd.Function.
+
+Model: a rank-R CP tensor [[A_0, ..., A_{N-1}]] fitted to the observed
+entries of a sparse tensor X (its nonzeros).  The loss is
+
+    L = 1/2 * sum_z (x_z - sum_r prod_m A_m[i_{m,z}, r])^2
+
+computed in FP64 from FP32 residuals.  Both directions run in libHiParTI's
+HIP kernels:
+
+  forward   ptiDevCOOCompletionForward: one pass over the nonzeros that
+            gathers the N factor rows of each entry, forms the rank sum, and
+            writes the residual e_z = x_z - y_z and the fixed-order FP64 sum
+            of e_z^2 / 2.
+  backward  dL/dA_n = -MTTKRP(E, n), where E is X's sparsity pattern carrying
+            the residuals as values: ptiDevMTTKRPCOOValues, the same COO
+            MTTKRP kernel as ptiMTTKRP (reference src/sptensor/mttkrp.c:58-91)
+            with the values replaced.
+
+The gradient is exact for the loss above (the sampled reconstruction is
+multilinear in the factors), so gradient checks against FP64 finite
+differences only see FP32 rounding.
+"""
+import ctypes as C
+
+import torch
+
+from ._lib import check, lib
+from .device import _check_factor, _order, _stream_ptr
+
+_vp = C.c_void_p
+
+
+class CPCompletionLoss(torch.autograd.Function):
+    """loss = CPCompletionLoss.apply(dev_coo, A_0, ..., A_{N-1}).
+
+    dev_coo: a DevCOO holding the observed entries.  A_m: contiguous float32
+    CUDA tensors of shape (ndims[m], R), 1 <= R <= 256.  Returns a 0-d
+    float64 tensor.  The residuals of the last forward are kept on the
+    context for backward."""
+
+    @staticmethod
+    def forward(ctx, dev, *facs):
+        N = dev.nmodes
+        if len(facs) != N:
+            raise ValueError("expected %d factor matrices, got %d" % (N, len(facs)))
+        R = facs[0].shape[1]
+        for m, f in enumerate(facs):
+            _check_factor(f, torch.float32, "factor %d" % m)
+            if f.shape[1] != R or f.shape[0] < dev.ndims[m]:
+                raise ValueError("factor %d must have shape (>= %d, %d)" % (m, dev.ndims[m], R))
+        device = facs[0].device
+        resid = torch.empty(dev.nnz, dtype=torch.float32, device=device)
+        loss = torch.empty((), dtype=torch.float64, device=device)
+        ptrs = (_vp * N)(*[f.data_ptr() for f in facs])
+        check(lib().ptiDevCOOCompletionForward(dev.handle, ptrs, R, R, _vp(resid.data_ptr()),
+                                               _vp(loss.data_ptr()), _stream_ptr()),
+              "ptiDevCOOCompletionForward")
+        ctx.dev = dev
+        ctx.save_for_backward(resid, *facs)
+        return loss
+
+    @staticmethod
+    def backward(ctx, gloss):
+        dev = ctx.dev
+        resid, *facs = ctx.saved_tensors
+        N = dev.nmodes
+        R = facs[0].shape[1]
+        scale = -gloss.to(torch.float32)
+        grads = [None]
+        for n in range(N):
+            if not ctx.needs_input_grad[n + 1]:
+                grads.append(None)
+                continue
+            out = torch.empty((facs[n].shape[0], R), dtype=torch.float32, device=facs[n].device)
+            ptrs = (_vp * (N + 1))(*[f.data_ptr() for f in facs], out.data_ptr())
+            check(lib().ptiDevMTTKRPCOOValues(dev.handle, _vp(resid.data_ptr()), ptrs, R, R,
+                                              _order(n, N, None), n, _stream_ptr()),
+                  "ptiDevMTTKRPCOOValues")
+            grads.append(out.mul_(scale))
+        return tuple(grads)
+
+
+def cp_completion_loss(dev, facs):
+    """Functional form: the FP64 CP-completion loss of factors `facs` on the
+    observed entries held by `dev` (a DevCOO), differentiable in `facs`."""
+    return CPCompletionLoss.apply(dev, *facs)
+
+
+def completion_residuals(dev, facs):
+    """Forward only: (loss, residuals) with residuals in the tensor's storage
+    order.  No autograd graph is recorded."""
+    N = dev.nmodes
+    R = facs[0].shape[1]
+    for m, f in enumerate(facs):
+        _check_factor(f, torch.float32, "factor %d" % m)
+    resid = torch.empty(dev.nnz, dtype=torch.float32, device=facs[0].device)
+    loss = torch.empty((), dtype=torch.float64, device=facs[0].device)
+    ptrs = (_vp * N)(*[f.data_ptr() for f in facs])
+    check(lib().ptiDevCOOCompletionForward(dev.handle, ptrs, R, R, _vp(resid.data_ptr()), _vp(loss.data_ptr()),
+                                           _stream_ptr()), "ptiDevCOOCompletionForward")
+    return loss, resid
+
+
+def mttkrp_values(dev, values, facs, mode, out=None):
+    """MTTKRP of X's sparsity pattern carrying `values` (nnz float32, storage
+    order) instead of X's own values."""
+    N = dev.nmodes
+    R = facs[0].shape[1]
+    for m, f in enumerate(facs):
+        _check_factor(f, torch.float32, "factor %d" % m)
+    if not (values.is_cuda and values.dtype == torch.float32 and values.is_contiguous()
+            and values.numel() == dev.nnz):
+        raise ValueError("values must be a contiguous float32 CUDA tensor of nnz elements")
+    if out is None:
+        out = torch.empty((dev.ndims[mode], R), dtype=torch.float32, device=facs[0].device)
+    ptrs = (_vp * (N + 1))(*[f.data_ptr() for f in facs], out.data_ptr())
+    check(lib().ptiDevMTTKRPCOOValues(dev.handle, _vp(values.data_ptr()), ptrs, R, R, _order(mode, N, None),
+                                      mode, _stream_ptr()), "ptiDevMTTKRPCOOValues")
+    return out
+
+
+__all__ = ["CPCompletionLoss", "cp_completion_loss", "completion_residuals", "mttkrp_values"]

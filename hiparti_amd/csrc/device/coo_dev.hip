@@ -22,6 +22,8 @@ struct ptiDevCOO {
     ptiIndex ndims[8] = {0};
     pti::dev::Buffer inds[8], vals;           // storage order
     pti::dev::Buffer sinds[8][8], svals[8];   // [target mode][mode] sorted copies
+    pti::dev::Buffer perm[8];                 // [target mode] sorted position -> storage position
+    pti::dev::Buffer partials;                // completion forward: per-workgroup sums
     bool ready[8] = {false};
     std::mutex mu;
 };
@@ -43,6 +45,20 @@ __global__ void __launch_bounds__(256) gather_kernel(T *__restrict__ dst, const 
                                                      const uint32_t *__restrict__ perm, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) dst[i] = src[perm[i]];
+}
+
+// 0.5 * sum of n partial sums, in a fixed order by one workgroup.
+__global__ void __launch_bounds__(256) half_sum_kernel(const double *partials, uint32_t n, double *out) {
+    __shared__ double s[256];
+    double t = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256) t += partials[i];
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = 0.5 * s[0];
 }
 
 inline dim3 grid_1d(uint64_t n) {
@@ -108,7 +124,8 @@ int prepare_mode(ptiDevCOO *t, ptiIndex mode, hipStream_t s) {
     for (uint32_t m = 0; m < N; ++m) PTI_HIP(t->sinds[mode][m].alloc(n * 4), M);
     PTI_HIP(t->svals[mode].alloc(n * 4), M);
     if (n) {
-        Buffer keys_out, ids_in, ids_out, tmp;
+        Buffer keys_out, ids_in, tmp;
+        Buffer &ids_out = t->perm[mode];
         PTI_HIP(keys_out.alloc(n * 4), M);
         PTI_HIP(ids_in.alloc(n * 4), M);
         PTI_HIP(ids_out.alloc(n * 4), M);
@@ -142,7 +159,7 @@ int prepare_mode(ptiDevCOO *t, ptiIndex mode, hipStream_t s) {
 }  // namespace
 
 int run_coo(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, const ptiIndex mats_order[],
-            ptiIndex mode, hipStream_t s, const char *M) {
+            ptiIndex mode, hipStream_t s, const char *M, const float *vals_override) {
     const uint32_t N = t->nmodes;
     if (mode >= N) PTI_FAIL(PTIERR_VALUE_ERROR, M, "mode out of range");
     if (R == 0 || stride < R) PTI_FAIL(PTIERR_SHAPE_MISMATCH, M, "bad rank or stride");
@@ -155,6 +172,8 @@ int run_coo(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, con
     COOArgs c;
     for (uint32_t m = 0; m < N; ++m) c.inds[m] = t->sinds[mode][m].as<uint32_t>();
     c.vals = t->svals[mode].as<float>();
+    c.vals_override = vals_override;
+    c.perm = t->perm[mode].as<uint32_t>();
     c.nnz = t->nnz;
     c.nchunks = (uint32_t)((t->nnz + CHUNK - 1) / CHUNK);
     const ptiIndex tile = 256;
@@ -194,7 +213,42 @@ void ptiDevCOOFree(ptiDevCOO *t) { delete t; }
 
 int ptiDevMTTKRPCOO(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, ptiIndex const mats_order[],
                     ptiIndex mode, void *stream) {
-    return run_coo(t, dmats, R, stride, mats_order, mode, static_cast<hipStream_t>(stream), "DevCOO MTTKRP");
+    return run_coo(t, dmats, R, stride, mats_order, mode, static_cast<hipStream_t>(stream), "DevCOO MTTKRP",
+                   nullptr);
+}
+
+int ptiDevMTTKRPCOOValues(ptiDevCOO *t, float const *dvals, float *const dmats[], ptiIndex R, ptiIndex stride,
+                          ptiIndex const mats_order[], ptiIndex mode, void *stream) {
+    if (!dvals && t->nnz) PTI_FAIL(PTIERR_VALUE_ERROR, "DevCOO MTTKRP values", "values must not be NULL");
+    return run_coo(t, dmats, R, stride, mats_order, mode, static_cast<hipStream_t>(stream), "DevCOO MTTKRP values",
+                   dvals);
+}
+
+int ptiDevCOOCompletionForward(ptiDevCOO *t, float *const dfacs[], ptiIndex R, ptiIndex stride, float *dresid,
+                               double *dloss, void *stream) {
+    static const char *M = "DevCOO Completion";
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint32_t N = t->nmodes;
+    Shape sh;
+    if (R == 0 || stride < R || !choose_shape(R, stride, stride, sh))
+        PTI_FAIL(PTIERR_VALUE_ERROR, M, "rank must be in [1, 256] and <= stride");
+    COOArgs c = {};
+    for (uint32_t m = 0; m < N; ++m) c.inds[m] = t->inds[m].as<uint32_t>();
+    c.vals = t->vals.as<float>();
+    c.nnz = t->nnz;
+    c.nchunks = (uint32_t)((t->nnz + CHUNK - 1) / CHUNK);
+    const uint32_t nblk = (c.nchunks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+    if (t->partials.bytes < (size_t)std::max<uint32_t>(nblk, 1) * sizeof(double))
+        PTI_HIP(t->partials.alloc((size_t)std::max<uint32_t>(nblk, 1) * sizeof(double)), M);
+    FactorArgs a = {};
+    for (uint32_t m = 0; m < N; ++m) a.fac[m] = dfacs[m];
+    a.R = R;
+    a.stride = stride;
+    a.out_stride = stride;
+    PTI_HIP(launch_completion(N, c, a, sh, dresid, t->partials.as<double>(), s), M);
+    hipLaunchKernelGGL(half_sum_kernel, dim3(1), dim3(256), 0, s, t->partials.as<double>(), nblk, dloss);
+    PTI_HIP(hipGetLastError(), M);
+    return 0;
 }
 
 int ptiRegisterSparseTensor(ptiSparseTensor const *X) {
@@ -235,7 +289,7 @@ int ptiMTTKRP(ptiSparseTensor const *const X, ptiMatrix *mats[], ptiIndex const 
     }
     StagedFactors sf;
     int rc = stage_factors(N, mats, mats_order, mode, sf, s, M);
-    if (!rc) rc = run_coo(t, sf.ptrs, mats[N]->ncols, mats[0]->stride, mats_order, mode, s, M);
+    if (!rc) rc = run_coo(t, sf.ptrs, mats[N]->ncols, mats[0]->stride, mats_order, mode, s, M, nullptr);
     if (!rc) rc = fetch_output(N, mats, mode, sf, s, M);
     delete owned;
     return rc;

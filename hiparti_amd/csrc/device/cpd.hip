@@ -26,7 +26,7 @@
 namespace pti {
 namespace dev {
 int run_coo(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, const ptiIndex mats_order[],
-            ptiIndex mode, hipStream_t s, const char *M);
+            ptiIndex mode, hipStream_t s, const char *M, const float *vals_override);
 }
 }  // namespace pti
 
@@ -217,7 +217,7 @@ extern "C" int ptiCpdAls(ptiSparseTensor const *const ptien, ptiIndex const rank
         for (ptiIndex m = 0; m < N && !rc; ++m) {
             order[0] = m;
             for (ptiIndex i = 1; i < N; ++i) order[i] = (m + i) % N;
-            rc = pti::dev::run_coo(dt, dptr, R, S, order.data(), m, s, M);
+            rc = pti::dev::run_coo(dt, dptr, R, S, order.data(), m, s, M, nullptr);
             if (rc) break;
             Mout.nrows = ptien->ndims[m];
             if (hipMemcpy(Mout.values, dout.ptr, (size_t)ptien->ndims[m] * S * sizeof(float), hipMemcpyDeviceToHost) !=

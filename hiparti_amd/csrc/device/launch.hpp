@@ -38,7 +38,9 @@ inline bool choose_shape(uint32_t R, uint32_t stride, uint32_t out_stride, Shape
 
 #define PTI_DECLARE_LAUNCHERS(N)                                                                       \
     hipError_t launch_hicoo_##N(const HiCOOArgs &h, const FactorArgs &a, Shape sh, bool bf16, hipStream_t s); \
-    hipError_t launch_coo_##N(const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s);
+    hipError_t launch_coo_##N(const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s);        \
+    hipError_t launch_completion_##N(const COOArgs &c, const FactorArgs &a, Shape sh, float *resid,    \
+                                     double *partials, hipStream_t s);
 
 PTI_DECLARE_LAUNCHERS(2)
 PTI_DECLARE_LAUNCHERS(3)
@@ -69,6 +71,13 @@ inline hipError_t launch_hicoo(uint32_t N, const HiCOOArgs &h, const FactorArgs 
 }
 inline hipError_t launch_coo(uint32_t N, const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s) {
 #define PTI_CALL(n) launch_coo_##n(c, a, sh, s)
+    PTI_DISPATCH_N(N, PTI_CALL)
+#undef PTI_CALL
+}
+
+inline hipError_t launch_completion(uint32_t N, const COOArgs &c, const FactorArgs &a, Shape sh, float *resid,
+                                    double *partials, hipStream_t s) {
+#define PTI_CALL(n) launch_completion_##n(c, a, sh, resid, partials, s)
     PTI_DISPATCH_N(N, PTI_CALL)
 #undef PTI_CALL
 }

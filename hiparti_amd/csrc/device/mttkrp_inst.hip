@@ -44,6 +44,27 @@ hipError_t coo_vg(const COOArgs &c, const FactorArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <int V, int G>
+hipError_t completion_vg(const COOArgs &c, const FactorArgs &a, float *resid, double *partials, hipStream_t s) {
+    if (c.nchunks == 0) return hipSuccess;
+    dim3 grid(grid_for_chunks(c.nchunks)), block(256);
+    hipLaunchKernelGGL((completion_forward_kernel<N, V, G>), grid, block, 0, s, c, a, resid, partials);
+    return hipGetLastError();
+}
+
+template <int V>
+hipError_t completion_v(const COOArgs &c, const FactorArgs &a, int G, float *resid, double *partials,
+                        hipStream_t s) {
+    switch (G) {
+        case 4: return completion_vg<V, 4>(c, a, resid, partials, s);
+        case 8: return completion_vg<V, 8>(c, a, resid, partials, s);
+        case 16: return completion_vg<V, 16>(c, a, resid, partials, s);
+        case 32: return completion_vg<V, 32>(c, a, resid, partials, s);
+        case 64: return completion_vg<V, 64>(c, a, resid, partials, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 template <int V>
 hipError_t hicoo_v(const HiCOOArgs &h, const FactorArgs &a, int G, bool bf16, hipStream_t s) {
     switch (G) {
@@ -84,6 +105,16 @@ hipError_t PTI_CAT(launch_coo_, PTI_NMODES)(const COOArgs &c, const FactorArgs &
         case 4: return coo_v<4>(c, a, sh.G, s);
         case 2: return coo_v<2>(c, a, sh.G, s);
         case 1: return coo_v<1>(c, a, sh.G, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t PTI_CAT(launch_completion_, PTI_NMODES)(const COOArgs &c, const FactorArgs &a, Shape sh, float *resid,
+                                                  double *partials, hipStream_t s) {
+    switch (sh.V) {
+        case 4: return completion_v<4>(c, a, sh.G, resid, partials, s);
+        case 2: return completion_v<2>(c, a, sh.G, resid, partials, s);
+        case 1: return completion_v<1>(c, a, sh.G, resid, partials, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -79,6 +79,11 @@ struct HiCOOArgs {
 struct COOArgs {
     const uint32_t *inds[MAX_MODES];  // coordinates per mode (sorted by the target mode)
     const float *vals;
+    // Optional replacement values in the tensor's storage order (for example
+    // CP-completion residuals); value z of the sorted copy is then
+    // vals_override[perm[z]].
+    const float *vals_override;
+    const uint32_t *perm;
     uint64_t nnz;
     uint32_t nchunks;
 };
@@ -193,7 +198,7 @@ __device__ __forceinline__ uint32_t stage_coo(const COOArgs &c, uint32_t chunk, 
     for (uint32_t zl = lane; zl < cnt; zl += 64) {
 #pragma unroll
         for (int m = 0; m < N; ++m) st.coord[m][skew(zl)] = c.inds[m][z0 + zl];
-        st.val[skew(zl)] = c.vals[z0 + zl];
+        st.val[skew(zl)] = c.vals_override ? c.vals_override[c.perm[z0 + zl]] : c.vals[z0 + zl];
     }
     wave_sync();
     return cnt;
@@ -441,6 +446,63 @@ __global__ void __launch_bounds__(256) coo_mttkrp_kernel(COOArgs c, FactorArgs a
     Staging<N> &st = st_all[wid];
     const uint32_t cnt = stage_coo<N>(c, chunk, st, lane);
     process_chunk<N, V, G, FT>(a, st, cnt, GlobalSink{a.out, a.out_stride}, lane);
+}
+
+// Sampled reconstruction at the nonzeros (CP completion forward):
+//   yhat_z = sum_r prod_m A_m[i_m][r],  resid_z = x_z - yhat_z,
+// and each workgroup's partial sum of resid_z^2 in FP64.  Storage order, one
+// wave per chunk of CHUNK nonzeros, G lanes x V ranks per nonzero; the rank
+// sum is a butterfly over the G lanes of a group.
+template <int N, int V, int G>
+__global__ void __launch_bounds__(256) completion_forward_kernel(COOArgs c, FactorArgs a, float *resid,
+                                                                 double *partials) {
+    __shared__ double wsum[WAVES_PER_BLOCK];
+    constexpr int NG = 64 / G;
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t g = (uint32_t)lane / G;
+    const uint32_t col = (uint32_t)(lane % G) * V;
+    const uint64_t chunk = (uint64_t)blockIdx.x * WAVES_PER_BLOCK + wid;
+    double sq = 0.0;
+    if (chunk < c.nchunks) {
+        const uint64_t z0 = chunk * CHUNK;
+        const uint64_t rem = c.nnz - z0;
+        const uint32_t cnt = rem < CHUNK ? (uint32_t)rem : (uint32_t)CHUNK;
+        const uint32_t iters = (cnt + NG - 1) / NG;  // the same for every group: shuffles stay uniform
+        for (uint32_t it = 0; it < iters; ++it) {
+            const uint32_t p = it * NG + g;
+            const bool valid = p < cnt;
+            const uint64_t z = z0 + (valid ? p : 0);
+            float part = 0.f;
+            if (valid && col < a.R) {
+                float prod[V], t[V];
+                FactorLoad<float, V>::load(a.fac[0], (uint64_t)c.inds[0][z] * a.stride + col, prod);
+#pragma unroll
+                for (int m = 1; m < N; ++m) {
+                    FactorLoad<float, V>::load(a.fac[m], (uint64_t)c.inds[m][z] * a.stride + col, t);
+#pragma unroll
+                    for (int k = 0; k < V; ++k) prod[k] *= t[k];
+                }
+#pragma unroll
+                for (int k = 0; k < V; ++k) part += prod[k];
+            }
+#pragma unroll
+            for (int off = G / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+            if (valid && (lane % G) == 0) {
+                const float e = c.vals[z] - part;
+                resid[z] = e;
+                sq += (double)e * (double)e;
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
+    if (lane == 0) wsum[wid] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < WAVES_PER_BLOCK; ++w) t += wsum[w];
+        partials[blockIdx.x] = t;
+    }
 }
 
 }  // namespace dev

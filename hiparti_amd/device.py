@@ -94,6 +94,7 @@ class DevCOO:
               "ptiDevCOOUpload")
         self.nmodes = coo.nmodes()
         self.ndims = coo.ndims()
+        self.nnz = coo.nnz()
 
     def prepare(self, mode, stream=None):
         check(lib().ptiDevCOOPrepareMode(self.handle, mode, _stream_ptr(stream)), "ptiDevCOOPrepareMode")

@@ -62,6 +62,20 @@ int ptiDevMTTKRPHiCOO_BF16(ptiDevHiCOO const *t, uint16_t const * const dfacs[],
                            float *dout, ptiIndex R, ptiIndex stride, ptiIndex out_stride,
                            ptiIndex const mats_order[], ptiIndex mode, void *stream);
 
+/* MTTKRP of a COO tensor whose values are replaced by dvals (device, nnz
+ * floats in the tensor's storage order), e.g. the residuals of a CP
+ * completion model: the adjoint of the sampled reconstruction below. */
+int ptiDevMTTKRPCOOValues(ptiDevCOO *t, float const *dvals, float * const dmats[], ptiIndex R,
+                          ptiIndex stride, ptiIndex const mats_order[], ptiIndex mode, void *stream);
+
+/* CP-completion forward at the observed entries (the tensor's nonzeros):
+ *   resid_z = x_z - sum_r prod_m A_m[i_m][r]   (dresid: nnz floats, storage order)
+ *   *dloss  = 0.5 * sum_z resid_z^2            (one double in device memory,
+ *                                               summed in a fixed order)
+ * dfacs: nmodes FP32 factor matrices with row stride `stride`; R <= 256. */
+int ptiDevCOOCompletionForward(ptiDevCOO *t, float * const dfacs[], ptiIndex R, ptiIndex stride,
+                               float *dresid, double *dloss, void *stream);
+
 /* Register a host HiCOO / COO tensor so that the drop-in ptiMTTKRPHiCOO /
  * ptiMTTKRP reuse a device copy instead of uploading the tensor per call.
  * The caller must unregister before modifying or freeing the host tensor. */
