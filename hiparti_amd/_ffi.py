@@ -64,6 +64,17 @@ class ptiSparseTensorHiCOO(C.Structure):
                 ("values", ptiValueVector)]
 
 
+class ptiRankMatrix(C.Structure):
+    _fields_ = [("nrows", ptiIndex), ("ncols", ptiElementIndex), ("cap", ptiIndex),
+                ("stride", ptiElementIndex), ("values", C.POINTER(ptiValue))]
+
+
+class ptiRankKruskalTensor(C.Structure):
+    _fields_ = [("nmodes", ptiIndex), ("rank", ptiElementIndex), ("ndims", C.POINTER(ptiIndex)),
+                ("lambda_", C.POINTER(ptiValue)), ("fit", C.c_double),
+                ("factors", C.POINTER(C.POINTER(ptiRankMatrix)))]
+
+
 class ptiKruskalTensor(C.Structure):
     _fields_ = [("nmodes", ptiIndex), ("rank", ptiIndex), ("ndims", C.POINTER(ptiIndex)),
                 ("lambda_", C.POINTER(ptiValue)), ("fit", C.c_double),
@@ -108,6 +119,25 @@ SIGNATURES = {
     "ptiNewKruskalTensor": (C.c_int, [_P(ptiKruskalTensor), ptiIndex, _P(ptiIndex), ptiIndex]),
     "ptiFreeKruskalTensor": (None, [_P(ptiKruskalTensor)]),
     "ptiCpdAls": (C.c_int, [_P(ptiSparseTensor), ptiIndex, ptiIndex, C.c_double, _P(ptiKruskalTensor)]),
+    "ptiOmpCpdAls": (C.c_int, [_P(ptiSparseTensor), ptiIndex, ptiIndex, C.c_double, C.c_int, C.c_int,
+                               _P(ptiKruskalTensor)]),
+    "ptiCpdAlsHiCOO": (C.c_int, [_P(ptiSparseTensorHiCOO), ptiIndex, ptiIndex, C.c_double,
+                                 _P(ptiRankKruskalTensor)]),
+    "ptiNewRankKruskalTensor": (C.c_int, [_P(ptiRankKruskalTensor), ptiIndex, _P(ptiIndex), ptiElementIndex]),
+    "ptiFreeRankKruskalTensor": (None, [_P(ptiRankKruskalTensor)]),
+    "ptiNewRankMatrix": (C.c_int, [_P(ptiRankMatrix), ptiIndex, ptiElementIndex]),
+    "ptiFreeRankMatrix": (None, [_P(ptiRankMatrix)]),
+    "ptiMatrixSolveNormals": (C.c_int, [ptiIndex, ptiIndex, _P(_P(ptiMatrix)), _P(ptiMatrix)]),
+    "ptiMatrixDotMulSeqTriangle": (C.c_int, [ptiIndex, ptiIndex, _P(_P(ptiMatrix))]),
+    "KruskalTensorFit": (C.c_double, [_P(ptiSparseTensor), _P(ptiValue), _P(_P(ptiMatrix)), _P(_P(ptiMatrix))]),
+    "KruskalTensorFrobeniusNormSquared": (C.c_double, [ptiIndex, _P(ptiValue), _P(_P(ptiMatrix))]),
+    "SparseKruskalTensorInnerProduct": (C.c_double, [ptiIndex, _P(ptiValue), _P(_P(ptiMatrix))]),
+    "GetFinalLambda": (None, [ptiIndex, ptiIndex, _P(_P(ptiMatrix)), _P(ptiValue)]),
+    "ptiOmpMTTKRP": (C.c_int, [_P(ptiSparseTensor), _P(_P(ptiMatrix)), _P(ptiIndex), ptiIndex, C.c_int]),
+    "ptiOmpMTTKRPHiCOO": (C.c_int, [_P(ptiSparseTensorHiCOO), _P(_P(ptiMatrix)), _P(ptiIndex), ptiIndex,
+                                    C.c_int, C.c_int]),
+    "ptiMTTKRPHiCOO_MatrixTiling": (C.c_int, [_P(ptiSparseTensorHiCOO), _P(_P(ptiRankMatrix)), _P(ptiIndex),
+                                              ptiIndex]),
     # synthetic inputs
     "ptiSynthSparseTensor": (C.c_int, [_P(ptiSparseTensor), ptiIndex, _P(ptiIndex), ptiNnzIndex,
                                        _P(C.c_int), _P(C.c_double), _P(C.c_int), C.c_int,
@@ -127,6 +157,16 @@ SIGNATURES = {
     "ptiDevCOOCompletionForward": (C.c_int, [_vp, _P(_vp), ptiIndex, ptiIndex, _vp, _vp, _vp]),
     "ptiDevMTTKRPHiCOO_BF16": (C.c_int, [_vp, _P(_vp), _vp, ptiIndex, ptiIndex, ptiIndex,
                                          _P(ptiIndex), ptiIndex, _vp]),
+    "ptiDevGram": (C.c_int, [_vp, ptiIndex, ptiIndex, ptiIndex, _vp, _vp]),
+    "ptiDevSolveNormals": (C.c_int, [_vp, ptiIndex, ptiIndex, ptiIndex, _vp, _vp, ptiIndex, ptiIndex,
+                                     _P(C.c_int), _vp]),
+    "ptiDevNormalizeColumns": (C.c_int, [_vp, ptiIndex, ptiIndex, ptiIndex, C.c_int, _vp, _vp]),
+    "ptiDevKruskalFit": (C.c_int, [_vp, ptiIndex, ptiIndex, _vp, _vp, _vp, ptiIndex, ptiIndex, C.c_double,
+                                   _P(C.c_double), _vp]),
+    "ptiDevCpdAlsHiCOO": (C.c_int, [_vp, _P(_vp), ptiIndex, ptiIndex, ptiIndex, C.c_double, ptiIndex, _vp,
+                                    _P(C.c_double), _P(C.c_double), _P(ptiIndex), _vp]),
+    "ptiDevCpdAlsCOO": (C.c_int, [_vp, _P(_vp), ptiIndex, ptiIndex, ptiIndex, C.c_double, _vp,
+                                  _P(C.c_double), _P(C.c_double), _P(ptiIndex), _vp]),
     "ptiRegisterHiCOO": (C.c_int, [_P(ptiSparseTensorHiCOO)]),
     "ptiUnregisterHiCOO": (C.c_int, [_P(ptiSparseTensorHiCOO)]),
     "ptiRegisterSparseTensor": (C.c_int, [_P(ptiSparseTensor)]),

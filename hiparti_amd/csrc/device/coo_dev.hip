@@ -194,6 +194,13 @@ int run_coo(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, con
     return 0;
 }
 
+void coo_view(const ptiDevCOO *t, uint32_t *nmodes, const ptiIndex **ndims, uint64_t *nnz, const float **vals) {
+    *nmodes = t->nmodes;
+    *ndims = t->ndims;
+    *nnz = t->nnz;
+    *vals = t->vals.as<float>();
+}
+
 }  // namespace dev
 }  // namespace pti
 

@@ -280,6 +280,10 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
         // alignment the vector width V relies on.
         Shape sh;
         if (!choose_shape(Rt, stride, out_stride, sh)) PTI_FAIL(PTIERR_VALUE_ERROR, M, "unsupported rank tile");
+        // The kernel stages factor-row offsets as 32-bit counts of V-wide vectors.
+        for (uint32_t i = 1; i < N; ++i)
+            if ((uint64_t)t->ndims[mats_order[i]] * (stride / sh.V) >= (1ull << 32))
+                PTI_FAIL(PTIERR_VALUE_ERROR, M, "factor matrix exceeds 2^32 row-vector slots");
         FactorArgs a;
         for (uint32_t m = 0; m < N; ++m) a.fac[m] = static_cast<const char *>(facs[m]) + (size_t)c0 * esz;
         a.out = out + c0;
@@ -291,6 +295,17 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
         PTI_HIP(launch_hicoo(N, hicoo_args(t, mode), a, sh, bf16, s), M);
     }
     return 0;
+}
+
+// Shape and values of a resident tensor, for callers that run several
+// kernels over it (CP-ALS).  vals is NULL when no mode copy is resident.
+void hicoo_view(const ptiDevHiCOO *t, uint32_t *nmodes, const ptiIndex **ndims, uint64_t *nnz, const float **vals) {
+    *nmodes = t->nmodes;
+    *ndims = t->ndims;
+    *nnz = t->nnz;
+    *vals = nullptr;
+    for (uint32_t m = 0; m < t->nmodes && !*vals; ++m)
+        if (t->mc[m].ready) *vals = t->mc[m].vals.as<float>();
 }
 
 }  // namespace dev

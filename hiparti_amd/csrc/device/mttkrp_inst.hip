@@ -27,11 +27,20 @@ hipError_t hicoo_vg(const HiCOOArgs &h, const FactorArgs &a, bool bf16, hipStrea
     } else {
         if (h.nunits == 0) return hipSuccess;
         const size_t lds = (size_t)(1u << h.sb_bits) * (G * V + 1) * sizeof(float);
-        dim3 grid(h.nunits), block(256);
-        if (bf16)
-            hipLaunchKernelGGL((hicoo_tile_kernel<N, V, G, uint16_t>), grid, block, lds, s, h, a);
-        else
-            hipLaunchKernelGGL((hicoo_tile_kernel<N, V, G, float>), grid, block, lds, s, h, a);
+        dim3 grid(h.nunits);
+        // 8 waves per workgroup when the staging of 8 chunks and the tile fit
+        // in 64 KB of LDS, else 4.
+        if (hicoo_stage_bytes<N>(8) + lds <= 65536) {
+            if (bf16)
+                hipLaunchKernelGGL((hicoo_tile_kernel<N, V, G, uint16_t, 8>), grid, dim3(512), lds, s, h, a);
+            else
+                hipLaunchKernelGGL((hicoo_tile_kernel<N, V, G, float, 8>), grid, dim3(512), lds, s, h, a);
+        } else {
+            if (bf16)
+                hipLaunchKernelGGL((hicoo_tile_kernel<N, V, G, uint16_t, 4>), grid, dim3(256), lds, s, h, a);
+            else
+                hipLaunchKernelGGL((hicoo_tile_kernel<N, V, G, float, 4>), grid, dim3(256), lds, s, h, a);
+        }
         return hipGetLastError();
     }
 }

@@ -162,15 +162,27 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Decode chunk `chunk` of a HiCOO mode copy into st (one wave, 64 lanes):
-// coordinate = (binds[first block + lblk] << sb_bits) + element offset.
-// Returns the number of nonzeros in the chunk (<= CHUNK).
+// HiCOO staging of one chunk, laid out for the slice walk: per position the
+// output row inside the block-row and the value (one 8-byte word), and for
+// each factor in product order (mats_order[1..N-1]) the row's offset in units
+// of the lane vector (coord * stride / V, checked on the host to fit 32 bits).
 template <int N>
-__device__ __forceinline__ uint32_t stage_hicoo(const HiCOOArgs &h, uint32_t chunk, Staging<N> &st, int lane) {
+struct HStage {
+    uint2 rv[CHUNK_SKEWED];               // .x output row (element offset), .y value bits
+    uint32_t off[N - 1][CHUNK_SKEWED];
+};
+
+// Decode chunk `chunk` of a HiCOO mode copy (one wave): coordinate of mode m
+// = (binds[first block + lblk] << sb_bits) + element offset.  Returns the
+// number of nonzeros in the chunk (<= CHUNK).
+template <int N, int V>
+__device__ __forceinline__ uint32_t stage_hicoo(const HiCOOArgs &h, const FactorArgs &a, uint32_t chunk,
+                                                HStage<N> &st, int lane) {
     constexpr int EW = (N + 3) / 4;
     const uint32_t z0 = h.chunk_z[chunk];
     const uint32_t cnt = h.chunk_z[chunk + 1] - z0;
     const uint32_t b0 = h.chunk_blk[chunk];
+    const uint32_t vstride = a.stride / V;
     wave_sync();
     for (uint32_t zl = lane; zl < cnt; zl += 64) {
         const uint32_t z = z0 + zl;
@@ -178,12 +190,13 @@ __device__ __forceinline__ uint32_t stage_hicoo(const HiCOOArgs &h, uint32_t chu
         uint32_t ew[EW];
 #pragma unroll
         for (int w = 0; w < EW; ++w) ew[w] = h.einds[(uint64_t)z * EW + w];
+        auto elem = [&](uint32_t m) { return (ew[m >> 2] >> ((m & 3) * 8)) & 0xFFu; };
+        st.rv[skew(zl)] = make_uint2(elem(a.mode), __float_as_uint(h.vals[z]));
 #pragma unroll
-        for (int m = 0; m < N; ++m) {
-            const uint32_t e = (ew[m >> 2] >> ((m & 3) * 8)) & 0xFFu;
-            st.coord[m][skew(zl)] = (h.binds[b * N + m] << h.sb_bits) + e;
+        for (int i = 1; i < N; ++i) {
+            const uint32_t m = a.order[i];
+            st.off[i - 1][skew(zl)] = ((h.binds[b * N + m] << h.sb_bits) + elem(m)) * vstride;
         }
-        st.val[skew(zl)] = h.vals[z];
     }
     wave_sync();
     return cnt;
@@ -242,20 +255,6 @@ struct GlobalSink {
         for (int k = 0; k < V; ++k) unsafeAtomicAdd(dst + k, val[k]);
     }
 };
-// TileSink: LDS atomics into the workgroup's output tile (HiCOO path); the
-// tile row is the row's offset inside its block-row.
-template <int TP>
-struct TileSink {
-    float *tile;
-    uint32_t mask;
-    template <int V>
-    __device__ __forceinline__ void add(uint32_t row, uint32_t col, const float *val) const {
-        float *dst = tile + (row & mask) * TP + col;
-#pragma unroll
-        for (int k = 0; k < V; ++k) atomicAdd(dst + k, val[k]);
-    }
-};
-
 // Walk a staged chunk (sorted by output row), reduce runs of equal output
 // rows across the lane groups and hand each run's sum to the sink once.
 template <int N, int V, int G, typename FT, typename Sink>
@@ -320,45 +319,64 @@ __device__ __forceinline__ void process_chunk(const FactorArgs &a, const Staging
     if (g == 0 && carry_row != INVALID_ROW && col < a.R) sink.template add<V>(carry_row, col, carry);
 }
 
+// Lane vector of V factor values as loaded from memory, and its conversion
+// to FP32 (BF16 bits are the high half of an FP32 word).
+template <typename FT, int V>
+struct Vec;
+template <> struct Vec<float, 4> { using T = float4; };
+template <> struct Vec<float, 2> { using T = float2; };
+template <> struct Vec<float, 1> { using T = float; };
+template <> struct Vec<uint16_t, 4> { using T = uint2; };
+template <> struct Vec<uint16_t, 2> { using T = uint32_t; };
+template <> struct Vec<uint16_t, 1> { using T = uint16_t; };
+
+__device__ __forceinline__ void to_float(const float4 &t, float *o) { o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w; }
+__device__ __forceinline__ void to_float(const float2 &t, float *o) { o[0] = t.x; o[1] = t.y; }
+__device__ __forceinline__ void to_float(const float &t, float *o) { o[0] = t; }
+__device__ __forceinline__ void to_float(const uint2 &t, float *o) {
+    o[0] = __uint_as_float(t.x << 16); o[1] = __uint_as_float(t.x & 0xFFFF0000u);
+    o[2] = __uint_as_float(t.y << 16); o[3] = __uint_as_float(t.y & 0xFFFF0000u);
+}
+__device__ __forceinline__ void to_float(const uint32_t &t, float *o) {
+    o[0] = __uint_as_float(t << 16); o[1] = __uint_as_float(t & 0xFFFF0000u);
+}
+__device__ __forceinline__ void to_float(const uint16_t &t, float *o) { o[0] = __uint_as_float((uint32_t)t << 16); }
+
 // Walk a staged chunk (sorted by output row) with each lane group taking a
 // contiguous slice of positions: equal rows are summed in registers and each
-// run is handed to the sink once.  No cross-lane traffic is needed; a row
-// that spans several slices is added once per slice.  The factor-row gathers
-// are software-pipelined D positions deep (slot j of the ring is consumed and
-// immediately refilled with the position D further on), so every lane keeps
-// D*(N-1) row loads in flight instead of waiting on each one.
-template <int N, int V, int G, int D, typename FT, typename Sink>
-__device__ __forceinline__ void process_slices(const FactorArgs &a, const Staging<N> &st, uint32_t cnt,
-                                               const Sink &sink, int lane) {
+// run is added to the LDS tile once (a row spanning several slices is added
+// once per slice).  The factor-row gathers are software-pipelined D
+// positions deep and issued unconditionally: positions past the slice are
+// redirected to position 0 of the chunk (always staged) and marked with an
+// invalid row, lanes past the rank read column 0, so the loads need no
+// branches and the compiler keeps D * (N - 1) of them in flight.  Lanes past
+// the rank add into tile columns >= R, which are never written out.
+template <int N, int V, int G, int D, typename FT>
+__device__ __forceinline__ void hicoo_slices(const FactorArgs &a, const HStage<N> &st, uint32_t cnt, float *tile,
+                                             uint32_t tp, int lane) {
+    using VT = typename Vec<FT, V>::T;
     constexpr int NG = 64 / G;
     const uint32_t g = (uint32_t)lane / G;
     const uint32_t col = (uint32_t)(lane % G) * V;
-    const bool active = col < a.R;
+    const uint32_t lcol = col < a.R ? col : 0u;
     const uint32_t per = (cnt + NG - 1) / NG;
     const uint32_t p0 = g * per;
     const uint32_t p1 = p0 + per < cnt ? p0 + per : cnt;
-    float fa[D][N - 1][V];
+    const VT *base[N - 1];
+#pragma unroll
+    for (int i = 0; i < N - 1; ++i)
+        base[i] = reinterpret_cast<const VT *>(static_cast<const FT *>(a.fac[a.order[i + 1]]) + lcol);
+    VT raw[D][N - 1];
     float vv[D];
     uint32_t rr[D];
     auto issue = [&](int j, uint32_t p) {
-        if (p < p1) {
-            const uint32_t li = skew(p);
-            rr[j] = st.coord[a.mode][li];
-            vv[j] = st.val[li];
-            if (active) {
+        const bool ok = p < p1;
+        const uint32_t li = skew(ok ? p : 0u);
+        const uint2 rv = st.rv[li];
+        rr[j] = ok ? rv.x : INVALID_ROW;
+        vv[j] = __uint_as_float(rv.y);
 #pragma unroll
-                for (int i = 1; i < N; ++i)
-                    FactorLoad<FT, V>::load(a.fac[a.order[i]], (uint64_t)st.coord[a.order[i]][li] * a.stride + col,
-                                            fa[j][i - 1]);
-            } else {
-#pragma unroll
-                for (int i = 1; i < N; ++i)
-#pragma unroll
-                    for (int k = 0; k < V; ++k) fa[j][i - 1][k] = 0.f;
-            }
-        } else {
-            rr[j] = INVALID_ROW;
-        }
+        for (int i = 0; i < N - 1; ++i) raw[j][i] = base[i][st.off[i][li]];
     };
 #pragma unroll
     for (int j = 0; j < D; ++j) issue(j, p0 + j);
@@ -366,64 +384,67 @@ __device__ __forceinline__ void process_slices(const FactorArgs &a, const Stagin
 #pragma unroll
     for (int k = 0; k < V; ++k) acc[k] = 0.f;
     uint32_t cur = INVALID_ROW;
-    for (uint32_t base = 0; base < per; base += D) {
+    for (uint32_t b = 0; b < per; b += D) {
 #pragma unroll
         for (int j = 0; j < D; ++j) {
+            float prod[V], t[V];
+            to_float(raw[j][0], prod);
+#pragma unroll
+            for (int k = 0; k < V; ++k) prod[k] *= vv[j];
+#pragma unroll
+            for (int i = 1; i < N - 1; ++i) {
+                to_float(raw[j][i], t);
+#pragma unroll
+                for (int k = 0; k < V; ++k) prod[k] *= t[k];
+            }
             const uint32_t row = rr[j];
-            float prod[V];
-            if (row != INVALID_ROW) {
+            issue(j, p0 + b + j + D);
+            const bool brk = row != cur;
+            if (brk && cur != INVALID_ROW) {
+                float *dst = tile + cur * tp + col;
 #pragma unroll
-                for (int k = 0; k < V; ++k) prod[k] = vv[j] * fa[j][0][k];
-#pragma unroll
-                for (int i = 1; i < N - 1; ++i)
-#pragma unroll
-                    for (int k = 0; k < V; ++k) prod[k] *= fa[j][i][k];
+                for (int k = 0; k < V; ++k) atomicAdd(dst + k, acc[k]);
             }
-            issue(j, p0 + base + j + D);
-            if (row != INVALID_ROW) {
-                if (row != cur) {
-                    if (cur != INVALID_ROW && active) sink.template add<V>(cur, col, acc);
 #pragma unroll
-                    for (int k = 0; k < V; ++k) acc[k] = prod[k];
-                    cur = row;
-                } else {
-#pragma unroll
-                    for (int k = 0; k < V; ++k) acc[k] += prod[k];
-                }
-            }
+            for (int k = 0; k < V; ++k) acc[k] = brk ? prod[k] : acc[k] + prod[k];
+            cur = row;
         }
     }
-    if (cur != INVALID_ROW && active) sink.template add<V>(cur, col, acc);
+    if (cur != INVALID_ROW) {
+        float *dst = tile + cur * tp + col;
+#pragma unroll
+        for (int k = 0; k < V; ++k) atomicAdd(dst + k, acc[k]);
+    }
 }
 
-// Pipeline depth of the gathers: deeper for low orders, where the ring of
-// D * (N - 1) * V loaded values stays small.
+// Pipeline depth of the gathers: D * (N - 1) vector loads in flight per lane.
 template <int N, int V>
 struct PipeDepth {
     static constexpr int value = (N - 1) * V <= 8 ? 4 : ((N - 1) * V <= 16 ? 2 : 1);
 };
 
-template <int N, int V, int G, typename FT>
-__global__ void __launch_bounds__(256) hicoo_tile_kernel(HiCOOArgs h, FactorArgs a) {
+// W waves per workgroup (W * 64 threads), one chunk per wave at a time.
+template <int N, int V, int G, typename FT, int W>
+__global__ void __launch_bounds__(W * 64) hicoo_tile_kernel(HiCOOArgs h, FactorArgs a) {
     extern __shared__ float tile[];  // (1 << sb_bits) rows x TP floats
-    __shared__ Staging<N> st_all[WAVES_PER_BLOCK];
+    __shared__ HStage<N> st_all[W];
     constexpr int GV = G * V;
     constexpr int TP = GV + 1;
+    constexpr int NT = W * 64;
     const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
     const uint32_t u = remap_block(blockIdx.x, gridDim.x);   // gridDim.x == nunits
     const uint32_t rows = 1u << h.sb_bits;
-    for (uint32_t i = tid; i < rows * TP; i += 256) tile[i] = 0.f;
+    for (uint32_t i = tid; i < rows * TP; i += NT) tile[i] = 0.f;
     __syncthreads();
     const uint32_t c0 = h.unit_chunk[u], c1 = h.unit_chunk[u + 1];
-    const TileSink<TP> sink{tile, rows - 1};
-    for (uint32_t c = c0 + wid; c < c1; c += WAVES_PER_BLOCK) {
-        const uint32_t cnt = stage_hicoo<N>(h, c, st_all[wid], lane);
-        process_slices<N, V, G, PipeDepth<N, V>::value, FT>(a, st_all[wid], cnt, sink, lane);
+    for (uint32_t c = c0 + wid; c < c1; c += W) {
+        const uint32_t cnt = stage_hicoo<N, V>(h, a, c, st_all[wid], lane);
+        if (cnt) hicoo_slices<N, V, G, PipeDepth<N, V>::value, FT>(a, st_all[wid], cnt, tile, TP, lane);
     }
     __syncthreads();
     const uint32_t base = h.unit_br[u] << h.sb_bits;
     const bool excl = h.unit_excl[u] != 0;
-    for (uint32_t i = tid; i < rows * GV; i += 256) {
+    for (uint32_t i = tid; i < rows * GV; i += NT) {
         const uint32_t r = i / GV, c = i % GV;
         const uint32_t orow = base + r;
         if (c >= a.R || orow >= h.nrows) continue;
@@ -435,6 +456,12 @@ __global__ void __launch_bounds__(256) hicoo_tile_kernel(HiCOOArgs h, FactorArgs
         else
             unsafeAtomicAdd(dst, v);
     }
+}
+
+// Static LDS of the HiCOO kernel's staging area for W waves.
+template <int N>
+constexpr size_t hicoo_stage_bytes(int W) {
+    return (size_t)W * sizeof(HStage<N>);
 }
 
 template <int N, int V, int G, typename FT>

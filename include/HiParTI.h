@@ -152,6 +152,20 @@ typedef struct {
     ptiMatrix **factors;
 } ptiKruskalTensor;
 
+/* CP decomposition result with ptiRankMatrix factors (structs.h:262-275). */
+typedef struct {
+    ptiIndex         nmodes;
+    ptiElementIndex  rank;
+    ptiIndex        *ndims;
+    ptiValue        *lambda;
+    double           fit;
+    ptiRankMatrix  **factors;
+} ptiRankKruskalTensor;
+
+/* OpenMP lock pool of the reference (structs.h:290-299).  Only passed
+ * through by pointer here; never dereferenced. */
+typedef struct ptiMutexPool ptiMutexPool;
+
 /* ---------------------------------------------------------------------------
  * Errors and timers (reference: include/includes/helper_funcs.h:29-50,
  * src/helper_funcs/error.c:27-118, src/helper_funcs/timer.c:34-100).
@@ -210,6 +224,37 @@ int  ptiDumpMatrix(ptiMatrix *mtx, FILE *fp);
 int  ptiMatrix2Norm(ptiMatrix * const A, ptiValue * const lambda);
 int  ptiMatrixMaxNorm(ptiMatrix * const A, ptiValue * const lambda);
 
+/* Normal-equation helpers of CP-ALS (reference: include/includes/matrices.h:47-65,
+ * src/matrix/matrix.c:340-385,556-572, src/matrix/matrix_solver.c:32-83).
+ * aTa[k] (k < nmodes) are R x R Grams of which only the row-major upper
+ * triangle (entries [i][j], j >= i: the triangle a column-major ssyrk 'L'
+ * writes) is read; aTa[nmodes] receives their Hadamard product over
+ * k != mode, made symmetric.
+ * ptiMatrixSolveNormals solves x V = m for every row m of rhs, in place, on
+ * the GPU in FP64: Cholesky, else LU with partial pivoting, else (V exactly
+ * singular) rhs is left unchanged.  Unlike the reference, aTa[nmodes] keeps
+ * V itself rather than its factorisation. */
+int  ptiMatrixDotMulSeqTriangle(ptiIndex const mode, ptiIndex const nmodes, ptiMatrix ** mats);
+int  ptiMatrixSolveNormals(ptiIndex const mode, ptiIndex const nmodes, ptiMatrix ** aTa, ptiMatrix * rhs);
+/* lambda[r] *= column-r 2-norm of every factor, which is normalised. */
+void GetFinalLambda(ptiIndex const rank, ptiIndex const nmodes, ptiMatrix ** mats, ptiValue * const lambda);
+
+/* Dense matrices with 8-bit ncols / stride (reference: matrices.h:68-89,
+ * src/matrix/rankmatrix.c).  ncols <= 248 keeps the stride representable. */
+int  ptiNewRankMatrix(ptiRankMatrix *mtx, ptiIndex const nrows, ptiElementIndex const ncols);
+/* Fills entry (i, j) with i + j + 1 (rankmatrix.c:69-77). */
+int  ptiRandomizeRankMatrix(ptiRankMatrix *mtx, ptiIndex const nrows, ptiElementIndex const ncols);
+int  ptiConstantRankMatrix(ptiRankMatrix *mtx, ptiValue const val);
+void ptiFreeRankMatrix(ptiRankMatrix *mtx);
+int  ptiDumpRankMatrix(ptiRankMatrix *mtx, FILE *fp);
+int  ptiRankMatrixDotMulSeqTriangle(ptiIndex const mode, ptiIndex const nmodes, ptiRankMatrix ** mats);
+int  ptiRankMatrix2Norm(ptiRankMatrix * const A, ptiValue * const lambda);
+int  ptiRankMatrixMaxNorm(ptiRankMatrix * const A, ptiValue * const lambda);
+void GetRankFinalLambda(ptiElementIndex const rank, ptiIndex const nmodes, ptiRankMatrix ** mats,
+                        ptiValue * const lambda);
+int  ptiRankMatrixSolveNormals(ptiIndex const mode, ptiIndex const nmodes, ptiRankMatrix ** aTa,
+                               ptiRankMatrix * rhs);
+
 /* ---------------------------------------------------------------------------
  * COO sparse tensors (reference: include/includes/sptensors.h:22-50).
  * ------------------------------------------------------------------------- */
@@ -266,6 +311,37 @@ int ptiCudaMTTKRP(ptiSparseTensor const * const X, ptiMatrix * mats[],
 int ptiCudaMTTKRPHiCOO(ptiSparseTensorHiCOO const * const hitsr, ptiMatrix ** const mats,
                        ptiIndex * const mats_order, ptiIndex const mode,
                        ptiNnzIndex const max_nnzb, int const impl_num);
+/* The reference's OpenMP and matrix-tiling variants (sptensors.h:184-203,
+ * :258-300).  They compute the same product and all run the GPU kernels:
+ * tk, tb, balanced, copy_mats and lock_pool are accepted and not used.
+ * ptiRankMatrix arguments share ptiMatrix's memory layout (row-major, 8-bit
+ * ncols and stride). */
+int ptiOmpMTTKRP(ptiSparseTensor const * const X, ptiMatrix * mats[],
+                 ptiIndex const mats_order[], ptiIndex const mode, const int tk);
+int ptiOmpMTTKRP_Reduce(ptiSparseTensor const * const X, ptiMatrix * mats[], ptiMatrix * copy_mats[],
+                        ptiIndex const mats_order[], ptiIndex const mode, const int tk);
+int ptiOmpMTTKRP_Lock(ptiSparseTensor const * const X, ptiMatrix * mats[],
+                      ptiIndex const mats_order[], ptiIndex const mode, const int tk,
+                      ptiMutexPool * lock_pool);
+int ptiMTTKRPHiCOO_MatrixTiling(ptiSparseTensorHiCOO const * const hitsr, ptiRankMatrix * mats[],
+                                ptiIndex const mats_order[], ptiIndex const mode);
+int ptiOmpMTTKRPHiCOO(ptiSparseTensorHiCOO const * const hitsr, ptiMatrix * mats[],
+                      ptiIndex const mats_order[], ptiIndex const mode, const int tk, const int tb);
+int ptiOmpMTTKRPHiCOO_MatrixTiling(ptiSparseTensorHiCOO const * const hitsr, ptiRankMatrix * mats[],
+                                   ptiIndex const mats_order[], ptiIndex const mode,
+                                   const int tk, const int tb);
+int ptiOmpMTTKRPHiCOO_MatrixTiling_Scheduled(ptiSparseTensorHiCOO const * const hitsr,
+                                             ptiRankMatrix * mats[], ptiIndex const mats_order[],
+                                             ptiIndex const mode, const int tk, const int tb,
+                                             int balanced);
+int ptiOmpMTTKRPHiCOO_MatrixTiling_Scheduled_Reduce(ptiSparseTensorHiCOO const * const hitsr,
+                                                    ptiRankMatrix * mats[], ptiRankMatrix * copy_mats[],
+                                                    ptiIndex const mats_order[], ptiIndex const mode,
+                                                    const int tk, const int tb, int balanced);
+int ptiOmpMTTKRPHiCOO_MatrixTiling_Scheduled_Reduce_Two(ptiSparseTensorHiCOO const * const hitsr,
+                                                        ptiRankMatrix * mats[], ptiRankMatrix * copy_mats[],
+                                                        ptiIndex const mats_order[], ptiIndex const mode,
+                                                        const int tk, const int tb);
 
 /* ---------------------------------------------------------------------------
  * Kruskal tensors and CPD-ALS (reference: include/includes/ktensors.h:23-39,
@@ -280,6 +356,42 @@ int    ptiDumpKruskalTensor(ptiKruskalTensor *ktsr, FILE *fp);
  * (reference: src/sptensor/cpd.c:163-208). */
 int    ptiCpdAls(ptiSparseTensor const * const ptien, ptiIndex const rank,
                  ptiIndex const niters, double const tol, ptiKruskalTensor * ktensor);
+/* Same algorithm; tk and use_reduce are accepted and not used (cpds.h:32-45). */
+int    ptiOmpCpdAls(ptiSparseTensor const * const ptien, ptiIndex const rank, ptiIndex const niters,
+                    double const tol, const int tk, const int use_reduce, ptiKruskalTensor * ktensor);
+int    ptiCudaCpdAls(ptiSparseTensor const * const ptien, ptiIndex const rank, ptiIndex const niters,
+                     double const tol, ptiKruskalTensor * ktensor);
+/* CP-ALS over the HiCOO MTTKRP, ptiRankMatrix factors, rank <= 248
+ * (cpds.h:46-60, src/sptensor/hicoo/cpd.c:153-204). */
+int    ptiCpdAlsHiCOO(ptiSparseTensorHiCOO const * const hitsr, ptiIndex const rank, ptiIndex const niters,
+                      double const tol, ptiRankKruskalTensor * ktensor);
+int    ptiOmpCpdAlsHiCOO(ptiSparseTensorHiCOO const * const hitsr, ptiIndex const rank,
+                         ptiIndex const niters, double const tol, const int tk, const int tb,
+                         int balanced, ptiRankKruskalTensor * ktensor);
+
+/* Fit of a Kruskal tensor against X (ktensors.h:27-39, ktensor.c:77-240):
+ * 1 - sqrt(||X||^2 + ||K||^2 - 2<X,K>) / ||X||, with ||K||^2 from the Grams
+ * ata[0..nmodes-1] (row-major upper triangles read, see
+ * ptiMatrixSolveNormals) and <X,K> from mats[nmodes-1] and the last MTTKRP
+ * output mats[nmodes].  FP64 sums.  The reference reads the triangle of the
+ * Grams that ssyrk leaves unwritten (ktensor.c:128-143); this reads the
+ * written one.  ata[nmodes] is overwritten with the Hadamard product. */
+double KruskalTensorFit(ptiSparseTensor const * const ptien, ptiValue const * const lambda,
+                        ptiMatrix ** mats, ptiMatrix ** ata);
+double KruskalTensorFrobeniusNormSquared(ptiIndex const nmodes, ptiValue const * const lambda,
+                                         ptiMatrix ** ata);
+double SparseKruskalTensorInnerProduct(ptiIndex const nmodes, ptiValue const * const lambda,
+                                       ptiMatrix ** mats);
+double KruskalTensorFitHiCOO(ptiSparseTensorHiCOO const * const hitsr, ptiValue const * const lambda,
+                             ptiRankMatrix ** mats, ptiRankMatrix ** ata);
+double KruskalTensorFrobeniusNormSquaredRank(ptiIndex const nmodes, ptiValue const * const lambda,
+                                             ptiRankMatrix ** ata);
+double SparseKruskalTensorInnerProductRank(ptiIndex const nmodes, ptiValue const * const lambda,
+                                           ptiRankMatrix ** mats);
+int    ptiNewRankKruskalTensor(ptiRankKruskalTensor *ktsr, ptiIndex nmodes, const ptiIndex ndims[],
+                               ptiElementIndex rank);
+void   ptiFreeRankKruskalTensor(ptiRankKruskalTensor *ktsr);
+int    ptiDumpRankKruskalTensor(ptiRankKruskalTensor *ktsr, FILE *fp);
 
 #ifdef __cplusplus
 }

@@ -76,6 +76,44 @@ int ptiDevMTTKRPCOOValues(ptiDevCOO *t, float const *dvals, float * const dmats[
 int ptiDevCOOCompletionForward(ptiDevCOO *t, float * const dfacs[], ptiIndex R, ptiIndex stride,
                                float *dresid, double *dloss, void *stream);
 
+/* FP64 dense step of CP-ALS on device data.  The reference does this work on
+ * the host with BLAS/LAPACK in FP32 (src/matrix/matrix_solver.c:32-83,
+ * src/matrix/matrix.c:340-572, src/ktensor/ktensor.c:77-240).  Matrices are
+ * row-major: factors FP32 with row stride `stride`, Grams FP64 R x R dense.
+ * R <= 256.  Each call synchronises `stream` before returning. */
+/* dG = A^T A (R x R, both triangles written), FP64 sums in a fixed order. */
+int ptiDevGram(float const *dA, ptiIndex nrows, ptiIndex R, ptiIndex stride, double *dG, void *stream);
+/* V = Hadamard product of dgrams[k] (nmodes contiguous R x R matrices),
+ * k != mode; dA = dM V^-1 row by row (Cholesky, else LU with partial
+ * pivoting; V exactly singular: dA = dM).  *status (host, may be NULL):
+ * 0 Cholesky, 1 LU, 2 singular.  dA and dM must not overlap. */
+int ptiDevSolveNormals(double const *dgrams, ptiIndex nmodes, ptiIndex mode, ptiIndex R, float const *dM,
+                       float *dA, ptiIndex nrows, ptiIndex stride, int *status, void *stream);
+/* Column norms into dlambda (R doubles) and division of the columns by them:
+ * 2-norm, or (max_norm != 0) the column maximum clamped to >= 1. */
+int ptiDevNormalizeColumns(float *dA, ptiIndex nrows, ptiIndex R, ptiIndex stride, int max_norm,
+                           double *dlambda, void *stream);
+/* fit = 1 - sqrt(normX2 + ||K||^2 - 2<X,K>) / sqrt(normX2), ||K||^2 from the
+ * full Grams and dlambda, <X,K> = sum_r lambda_r sum_i dlast[i][r] dM[i][r]. */
+int ptiDevKruskalFit(double const *dgrams, ptiIndex nmodes, ptiIndex R, double const *dlambda,
+                     float const *dlast, float const *dM, ptiIndex nrows_last, ptiIndex stride,
+                     double normX2, double *fit, void *stream);
+
+/* CP-ALS entirely on the device, starting from the caller's factors
+ * dfacs[0..nmodes-1] (FP32, row stride `stride`, updated in place).  The
+ * iteration is the reference's (src/sptensor/cpd.c:32-160) with the dense
+ * step above.  dlambda: R doubles (device, may be NULL).  *fit (host): last
+ * fit; fit_trace (host, may be NULL): niters entries, one per iteration run;
+ * *iters_done (host, may be NULL).  Stops after niters or when the fit
+ * changes by less than tol.  For HiCOO, the first bf16_iters iterations
+ * gather BF16 copies of the factors (FP32 accumulation, FP32 masters). */
+int ptiDevCpdAlsHiCOO(ptiDevHiCOO const *t, float * const dfacs[], ptiIndex R, ptiIndex stride,
+                      ptiIndex niters, double tol, ptiIndex bf16_iters, double *dlambda, double *fit,
+                      double *fit_trace, ptiIndex *iters_done, void *stream);
+int ptiDevCpdAlsCOO(ptiDevCOO *t, float * const dfacs[], ptiIndex R, ptiIndex stride, ptiIndex niters,
+                    double tol, double *dlambda, double *fit, double *fit_trace, ptiIndex *iters_done,
+                    void *stream);
+
 /* Register a host HiCOO / COO tensor so that the drop-in ptiMTTKRPHiCOO /
  * ptiMTTKRP reuse a device copy instead of uploading the tensor per call.
  * The caller must unregister before modifying or freeing the host tensor. */

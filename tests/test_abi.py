@@ -66,6 +66,11 @@ int main(void) {
   P(ptiSparseTensorHiCOO, bptr) P(ptiSparseTensorHiCOO, binds) P(ptiSparseTensorHiCOO, einds) P(ptiSparseTensorHiCOO, values)
   S(ptiKruskalTensor) P(ptiKruskalTensor, nmodes) P(ptiKruskalTensor, rank) P(ptiKruskalTensor, ndims)
   P(ptiKruskalTensor, lambda) P(ptiKruskalTensor, fit) P(ptiKruskalTensor, factors)
+  S(ptiRankMatrix) P(ptiRankMatrix, nrows) P(ptiRankMatrix, ncols) P(ptiRankMatrix, cap) P(ptiRankMatrix, stride)
+  P(ptiRankMatrix, values)
+  S(ptiRankKruskalTensor) P(ptiRankKruskalTensor, nmodes) P(ptiRankKruskalTensor, rank)
+  P(ptiRankKruskalTensor, ndims) P(ptiRankKruskalTensor, lambda) P(ptiRankKruskalTensor, fit)
+  P(ptiRankKruskalTensor, factors)
   S(ptiValueVector) S(ptiIndexVector) S(ptiElementIndexVector) S(ptiBlockIndexVector) S(ptiNnzIndexVector)
   return 0;
 }
@@ -89,6 +94,7 @@ def ctypes_layout():
     res = {}
     for name, T in (("ptiMatrix", F.ptiMatrix), ("ptiSparseTensor", F.ptiSparseTensor),
                     ("ptiSparseTensorHiCOO", F.ptiSparseTensorHiCOO), ("ptiKruskalTensor", F.ptiKruskalTensor),
+                    ("ptiRankMatrix", F.ptiRankMatrix), ("ptiRankKruskalTensor", F.ptiRankKruskalTensor),
                     ("ptiValueVector", F.ptiValueVector), ("ptiIndexVector", F.ptiIndexVector),
                     ("ptiElementIndexVector", F.ptiElementIndexVector),
                     ("ptiBlockIndexVector", F.ptiBlockIndexVector), ("ptiNnzIndexVector", F.ptiNnzIndexVector)):

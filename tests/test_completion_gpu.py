@@ -11,7 +11,7 @@ entries and its gradient dL/dA_n = -MTTKRP(residuals, n) are checked:
     residuals, i.e. ptiMTTKRP (reference src/sptensor/mttkrp.c:58-91) on the
     residual tensor;
   - ptiDevMTTKRPCOOValues fed the tensor's own values against
-    ptiDevMTTKRPCOO (bit-identical: the same kernel, the same order).
+    ptiDevMTTKRPCOO and the FP64 oracle.
 """
 import numpy as np
 import pytest
@@ -144,14 +144,17 @@ def test_gradient_is_negative_mttkrp_of_residuals(ndims, nnz, R, oracle):
         assert rel_fro(params[n].grad.cpu().numpy(), ref) <= TOL, n
 
 
-def test_values_override_with_own_values_is_bit_identical():
+def test_values_override_with_own_values_matches_plain_mttkrp(oracle):
+    # Rows that cross a chunk boundary are accumulated with atomics, so the
+    # two calls agree to rounding, not bit for bit.
     coo, dev, inds, vals, facs, tfacs = setup((120, 90, 70), 20000, 32, seed=9)
     tv = torch.from_numpy(vals).cuda()
     for n in range(3):
-        a = mttkrp_values(dev, tv, tfacs, n)
-        b = dev.mttkrp(tfacs, n)
-        torch.cuda.synchronize()
-        assert torch.equal(a, b), n
+        a = mttkrp_values(dev, tv, tfacs, n).cpu().numpy()
+        b = dev.mttkrp(tfacs, n).cpu().numpy()
+        ref = oracle.mttkrp(inds, vals, facs, n, kind="f64")
+        assert rel_fro(a, b) <= 1e-6, n
+        assert rel_fro(a, ref) <= TOL, n
 
 
 def test_partial_gradients_and_descent():
