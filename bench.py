@@ -16,14 +16,22 @@ the HiCOO arrays) divided by the measured average duration of one mode's
 MTTKRP (HIP events on the launch stream; this includes the output zero
 fill), against 8 TB/s; the BF16-factor form's time; and the oracle's
 OpenMP HiCOO MTTKRP (the reference's superblock-parallel scheme) timed on a
-bounded sample of the same tensor.
+bounded sample of the same tensor.  roofline.traffic is the HBM bytes per
+launch from the committed rocprofv3 counter summary of this kernel (null if
+that summary was taken on another kernel version).
 
-    python bench.py [--steps K] [--warmup W] [--nnz NNZ] [--no-cpu]
+The "cpd" object is the CPD-ALS leg: an order-5 tensor 1M x 500k x 100k x
+1k x 24 with 150M nonzeros, rank 16, 20 iterations on the device from one
+seeded set of factors, all-FP32 versus the BF16-then-FP32 schedule (s/iter
+and final fit of each).
+
+    python bench.py [--steps K] [--warmup W] [--nnz NNZ] [--no-cpu] [--no-cpd]
 """
 import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -46,6 +54,11 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-bf16", action="store_true")
     p.add_argument("--cpu-sample-nnz", type=float, default=8e6)
+    p.add_argument("--no-cpd", action="store_true", help="skip the CPD-ALS leg")
+    p.add_argument("--cpd-nnz", type=float, default=150e6)
+    p.add_argument("--cpd-rank", type=int, default=16)
+    p.add_argument("--cpd-iters", type=int, default=20)
+    p.add_argument("--cpd-bf16-iters", type=int, default=10)
     p.add_argument("--copy-calibration", type=float, default=0.0,
                    help="only copy a GiB-sized float buffer 3 times (HBM counter calibration) and exit")
     return p.parse_args()
@@ -62,6 +75,25 @@ def copy_calibration(gib):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes_per_copy": n * 4, "copies": 3}), flush=True)
+
+
+KERNEL = "hicoo_tile_kernel<3, 4, 8, float, 8>"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hicoo_tile_kernel_zipf100M_R32_counters.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the FP32 HiCOO kernel on this workload, from
+    the committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE,
+    calibrated on a plain copy), when it was taken on the current kernel."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    kern = re.search(r"^kernel: (.*)$", text, re.M)
+    val = re.search(r"^traffic_bytes_per_launch: (\d+)$", text, re.M)
+    if not kern or not val or kern.group(1).strip() != KERNEL:
+        return None
+    return int(val.group(1))
 
 
 def log(*a):
@@ -102,6 +134,55 @@ def cpu_baseline(h, ndims, facs_np, R, sample_nnz, nthreads):
         total += (time.perf_counter() - t0) / 2
     per_nnz = total / z_end
     return per_nnz * int(a.nnz) * 1e3, z_end, k_end
+
+
+def cpd_leg(args):
+    """CP-ALS on the device: order-5 tensor 1M x 500k x 100k x 1k x 24,
+    Zipf(1.1) indices on modes 0-2 and uniform on modes 3-4, values U(0,1),
+    HiCOO sb=7 sk=14 sc=14, rank 16, a fixed number of iterations from the
+    same seeded U(0,1) factors: all-FP32 MTTKRPs, then the schedule whose
+    first iterations gather BF16 factor copies.  Returns s/iter and the final
+    fits of both runs."""
+    import torch
+    from hiparti_amd import COOTensor
+    from hiparti_amd import cpd as CP
+    from hiparti_amd.device import DevHiCOO
+    ndims = [1000000, 500000, 100000, 1000, 24]
+    nnz = int(args.cpd_nnz)
+    R = args.cpd_rank
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.time()
+    X = COOTensor.synthetic(ndims, nnz, dist=[1, 1, 1, 0, 0], zipf_s=[1.1, 1.1, 1.1, 0, 0],
+                            permute=[0, 0, 0, 0, 0], seed=args.seed + 4, nthreads=nthreads)
+    H = X.toHiCOO(7, 14, 14, nthreads=nthreads)
+    X.free()
+    D = DevHiCOO(H)
+    nb = H.nblocks()
+    H.free()
+    torch.cuda.synchronize()
+    log("cpd tensor: %d nnz, nb=%d, ready in %.1f s" % (nnz, nb, time.time() - t0))
+    init = CP.init_factors(ndims, R, "uniform", seed=args.seed)
+    # One warm-up iteration, then the timed runs from the same factors.
+    CP.cp_als(D, R, niters=1, tol=0.0, factors=[f.clone() for f in init])
+    runs = {}
+    for name, bf in (("fp32", 0), ("bf16_fp32", args.cpd_bf16_iters)):
+        facs = [f.clone() for f in init]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = CP.cp_als(D, R, niters=args.cpd_iters, tol=0.0, factors=facs, bf16_iters=bf)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        runs[name] = (sec / max(res.iters, 1), res.fit, res.iters)
+        log("cpd %s: %.4f s/iter, fit %.6f after %d iterations" % (name, runs[name][0], res.fit, res.iters))
+    del D
+    torch.cuda.empty_cache()
+    return {"cpd": {"workload": "CP-ALS order-5 1M x 500k x 100k x 1k x 24, %d nnz, Zipf(1.1) modes 0-2, "
+                                "uniform modes 3-4, R=%d, %d iterations, HiCOO sb=7 sk=14" % (nnz, R, args.cpd_iters),
+                    "nblocks": nb,
+                    "fp32_s_per_iter": round(runs["fp32"][0], 5), "fp32_fit": runs["fp32"][1],
+                    "bf16_iters": args.cpd_bf16_iters,
+                    "bf16_fp32_s_per_iter": round(runs["bf16_fp32"][0], 5), "bf16_fp32_fit": runs["bf16_fp32"][1],
+                    "fit_delta": abs(runs["fp32"][1] - runs["bf16_fp32"][1])}}
 
 
 def main():
@@ -195,8 +276,8 @@ def main():
     avg_ms = float(np.mean(per_mode))
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "hicoo_mttkrp_kernel<3,4,8,float> (+ output zero fill)",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+                "kernel": KERNEL + " (+ output zero fill)",
                 "alg_bytes_per_launch": int(alg_bytes)}
     extra = {"per_mode_ms": [round(x, 4) for x in per_mode], "nblocks": nb, "nnz": nnz, "rank": R}
     if bf16_ms is not None:
@@ -206,9 +287,14 @@ def main():
         extra["bf16_hbm_frac"] = round(bf_bytes / (np.mean(bf16_per_mode) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
 
     cpu = None
+    cpu_args = (H, ndims, facs_np, R)
+    if not args.no_cpd:
+        del D, dfacs, outs, bfacs
+        torch.cuda.empty_cache()
+        extra.update(cpd_leg(args))
     if not args.no_cpu:
         t0 = time.time()
-        cpu_ms, z_s, k_s = cpu_baseline(H, ndims, facs_np, R, args.cpu_sample_nnz, nthreads)
+        cpu_ms, z_s, k_s = cpu_baseline(*cpu_args, args.cpu_sample_nnz, nthreads)
         cpu = {"value": round(cpu_ms, 1), "unit": "ms per step (3-mode HiCOO MTTKRP, extrapolated per nonzero)",
                "cores": nthreads, "kind": "port",
                "sample": "oracle OpenMP HiCOO MTTKRP (tk=%d, tb=1) on the first %d superblocks = %d nonzeros of "
@@ -216,7 +302,7 @@ def main():
         log("cpu baseline %.1f s" % (time.time() - t0))
 
     line = {
-        "metric": "HiCOO MTTKRP ms per 3-mode sweep (fp32) & % HBM peak",
+        "metric": "HiCOO MTTKRP ms & % HBM peak (fp32 vs bf16); CPD-ALS s/iter at equal fit",
         "value": round(ms_per_step, 4),
         "unit": "ms",
         "n_gpus": 1,
