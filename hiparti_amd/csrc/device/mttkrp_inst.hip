@@ -16,6 +16,9 @@ namespace dev {
 namespace {
 constexpr int N = PTI_NMODES;
 
+constexpr size_t LDS_PER_CU = 160 * 1024;   // gfx950
+constexpr size_t MAX_WAVES_PER_CU = 32;
+
 inline uint32_t grid_for_chunks(uint32_t nchunks) {
     return (nchunks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
 }
@@ -28,9 +31,18 @@ hipError_t hicoo_vg(const HiCOOArgs &h, const FactorArgs &a, bool bf16, hipStrea
         if (h.nunits == 0) return hipSuccess;
         const size_t lds = (size_t)(1u << h.sb_bits) * (G * V + 1) * sizeof(float);
         dim3 grid(h.nunits);
-        // 8 waves per workgroup when the staging of 8 chunks and the tile fit
-        // in 64 KB of LDS, else 4.
-        if (hicoo_stage_bytes<N>(8) + lds <= 65536) {
+        // 8 or 4 waves per workgroup, whichever keeps more waves resident per
+        // CU within gfx950's 160 KB of LDS (ties: 8).
+        auto resident = [&](int W) -> size_t {
+            const size_t per_wg = hicoo_stage_bytes<N, V, G>(W) + lds;
+            if (per_wg > LDS_PER_CU) return 0;
+            size_t wgs = LDS_PER_CU / per_wg;
+            if (wgs * W > MAX_WAVES_PER_CU) wgs = MAX_WAVES_PER_CU / W;
+            return wgs * W;
+        };
+        const size_t r8 = resident(8), r4 = resident(4);
+        if (r8 == 0 && r4 == 0) return hipErrorInvalidValue;
+        if (r8 >= r4) {
             if (bf16)
                 hipLaunchKernelGGL((hicoo_tile_kernel<N, V, G, uint16_t, 8>), grid, dim3(512), lds, s, h, a);
             else

@@ -48,6 +48,7 @@ namespace dev {
 constexpr int CHUNK = 256;          // nonzeros per wave chunk
 constexpr int WAVES_PER_BLOCK = 4;  // waves (chunks) per 256-thread workgroup
 constexpr uint32_t INVALID_ROW = 0xFFFFFFFFu;
+constexpr uint32_t DUMMY_ROW = 0xFFu;   // HiCOO staging: 8-bit rows, sb_bits <= 7
 constexpr int MAX_MODES = 8;
 
 struct FactorArgs {
@@ -162,27 +163,41 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// HiCOO staging of one chunk, laid out for the slice walk: per position the
-// output row inside the block-row and the value (one 8-byte word), and for
-// each factor in product order (mats_order[1..N-1]) the row's offset in units
-// of the lane vector (coord * stride / V, checked on the host to fit 32 bits).
-template <int N>
+// HiCOO staging of one chunk, laid out for the slice walk.  The NG lane
+// groups each take a contiguous slice of per = ceil(cnt / NG) positions;
+// local position j of group g is stored at entry j * NG + g, so the groups of
+// a wave read NG consecutive entries (distinct banks) and a group's next
+// position is NG entries further on.  Per entry: the output row inside the
+// block-row (the target mode's element offset) and the value in one 8-byte
+// word, and for each factor in product order (mats_order[1..N-1]) the row's
+// offset in units of the lane vector (coord * stride / V, checked on the
+// host to fit 32 bits).  Entries past a group's slice, up to per + 2 * D
+// positions, are dummies (invalid row, offset 0) so that the pipelined walk
+// can read and gather ahead without bounds tests.
+template <int NG, int D>
+constexpr int hstage_entries() {
+    return CHUNK + NG + 2 * D * NG;
+}
+
+template <int N, int E>
 struct HStage {
-    uint2 rv[CHUNK_SKEWED];               // .x output row (element offset), .y value bits
-    uint32_t off[N - 1][CHUNK_SKEWED];
+    uint32_t off[N - 1][E];
+    float val[E];
+    uint8_t row[E];             // output row inside the block-row; 0xFF: dummy
 };
 
 // Decode chunk `chunk` of a HiCOO mode copy (one wave): coordinate of mode m
 // = (binds[first block + lblk] << sb_bits) + element offset.  Returns the
 // number of nonzeros in the chunk (<= CHUNK).
-template <int N, int V>
+template <int N, int V, int NG, int D, int E>
 __device__ __forceinline__ uint32_t stage_hicoo(const HiCOOArgs &h, const FactorArgs &a, uint32_t chunk,
-                                                HStage<N> &st, int lane) {
+                                                HStage<N, E> &st, int lane) {
     constexpr int EW = (N + 3) / 4;
     const uint32_t z0 = h.chunk_z[chunk];
     const uint32_t cnt = h.chunk_z[chunk + 1] - z0;
     const uint32_t b0 = h.chunk_blk[chunk];
     const uint32_t vstride = a.stride / V;
+    const uint32_t per = (cnt + NG - 1) / NG;
     wave_sync();
     for (uint32_t zl = lane; zl < cnt; zl += 64) {
         const uint32_t z = z0 + zl;
@@ -191,11 +206,26 @@ __device__ __forceinline__ uint32_t stage_hicoo(const HiCOOArgs &h, const Factor
 #pragma unroll
         for (int w = 0; w < EW; ++w) ew[w] = h.einds[(uint64_t)z * EW + w];
         auto elem = [&](uint32_t m) { return (ew[m >> 2] >> ((m & 3) * 8)) & 0xFFu; };
-        st.rv[skew(zl)] = make_uint2(elem(a.mode), __float_as_uint(h.vals[z]));
+        const uint32_t g = zl / per;
+        const uint32_t e = (zl - g * per) * NG + g;
+        st.row[e] = (uint8_t)elem(a.mode);
+        st.val[e] = h.vals[z];
 #pragma unroll
         for (int i = 1; i < N; ++i) {
             const uint32_t m = a.order[i];
-            st.off[i - 1][skew(zl)] = ((h.binds[b * N + m] << h.sb_bits) + elem(m)) * vstride;
+            st.off[i - 1][e] = ((h.binds[b * N + m] << h.sb_bits) + elem(m)) * vstride;
+        }
+    }
+    // Dummies: local positions at or past each group's slice length.
+    const uint32_t span = (per + 2 * D) * NG;
+    for (uint32_t e = lane; e < span; e += 64) {
+        const uint32_t g = e % NG, j = e / NG;
+        const uint32_t len = g * per < cnt ? (cnt - g * per < per ? cnt - g * per : per) : 0u;
+        if (j >= len) {
+            st.row[e] = DUMMY_ROW;
+            st.val[e] = 0.f;
+#pragma unroll
+            for (int i = 0; i < N - 1; ++i) st.off[i][e] = 0u;
         }
     }
     wave_sync();
@@ -346,13 +376,13 @@ __device__ __forceinline__ void to_float(const uint16_t &t, float *o) { o[0] = _
 // contiguous slice of positions: equal rows are summed in registers and each
 // run is added to the LDS tile once (a row spanning several slices is added
 // once per slice).  The factor-row gathers are software-pipelined D
-// positions deep and issued unconditionally: positions past the slice are
-// redirected to position 0 of the chunk (always staged) and marked with an
-// invalid row, lanes past the rank read column 0, so the loads need no
-// branches and the compiler keeps D * (N - 1) of them in flight.  Lanes past
-// the rank add into tile columns >= R, which are never written out.
-template <int N, int V, int G, int D, typename FT>
-__device__ __forceinline__ void hicoo_slices(const FactorArgs &a, const HStage<N> &st, uint32_t cnt, float *tile,
+// positions deep and issued unconditionally (the staging pads every slice
+// with dummy entries that gather row 0 and carry an invalid row), and lanes
+// past the rank read column 0, so the loop has no branches around loads and
+// keeps D * (N - 1) of them in flight.  Lanes past the rank add into tile
+// columns >= R, which are never written out.
+template <int N, int V, int G, int D, typename FT, int E>
+__device__ __forceinline__ void hicoo_slices(const FactorArgs &a, const HStage<N, E> &st, uint32_t cnt, float *tile,
                                              uint32_t tp, int lane) {
     using VT = typename Vec<FT, V>::T;
     constexpr int NG = 64 / G;
@@ -360,8 +390,6 @@ __device__ __forceinline__ void hicoo_slices(const FactorArgs &a, const HStage<N
     const uint32_t col = (uint32_t)(lane % G) * V;
     const uint32_t lcol = col < a.R ? col : 0u;
     const uint32_t per = (cnt + NG - 1) / NG;
-    const uint32_t p0 = g * per;
-    const uint32_t p1 = p0 + per < cnt ? p0 + per : cnt;
     const VT *base[N - 1];
 #pragma unroll
     for (int i = 0; i < N - 1; ++i)
@@ -369,21 +397,18 @@ __device__ __forceinline__ void hicoo_slices(const FactorArgs &a, const HStage<N
     VT raw[D][N - 1];
     float vv[D];
     uint32_t rr[D];
-    auto issue = [&](int j, uint32_t p) {
-        const bool ok = p < p1;
-        const uint32_t li = skew(ok ? p : 0u);
-        const uint2 rv = st.rv[li];
-        rr[j] = ok ? rv.x : INVALID_ROW;
-        vv[j] = __uint_as_float(rv.y);
+    auto issue = [&](int j, uint32_t e) {
+        rr[j] = st.row[e];
+        vv[j] = st.val[e];
 #pragma unroll
-        for (int i = 0; i < N - 1; ++i) raw[j][i] = base[i][st.off[i][li]];
+        for (int i = 0; i < N - 1; ++i) raw[j][i] = base[i][st.off[i][e]];
     };
 #pragma unroll
-    for (int j = 0; j < D; ++j) issue(j, p0 + j);
+    for (int j = 0; j < D; ++j) issue(j, j * NG + g);
     float acc[V];
 #pragma unroll
     for (int k = 0; k < V; ++k) acc[k] = 0.f;
-    uint32_t cur = INVALID_ROW;
+    uint32_t cur = DUMMY_ROW;
     for (uint32_t b = 0; b < per; b += D) {
 #pragma unroll
         for (int j = 0; j < D; ++j) {
@@ -398,9 +423,9 @@ __device__ __forceinline__ void hicoo_slices(const FactorArgs &a, const HStage<N
                 for (int k = 0; k < V; ++k) prod[k] *= t[k];
             }
             const uint32_t row = rr[j];
-            issue(j, p0 + b + j + D);
+            issue(j, (b + j + D) * NG + g);
             const bool brk = row != cur;
-            if (brk && cur != INVALID_ROW) {
+            if (brk && cur != DUMMY_ROW) {
                 float *dst = tile + cur * tp + col;
 #pragma unroll
                 for (int k = 0; k < V; ++k) atomicAdd(dst + k, acc[k]);
@@ -410,7 +435,7 @@ __device__ __forceinline__ void hicoo_slices(const FactorArgs &a, const HStage<N
             cur = row;
         }
     }
-    if (cur != INVALID_ROW) {
+    if (cur != DUMMY_ROW) {
         float *dst = tile + cur * tp + col;
 #pragma unroll
         for (int k = 0; k < V; ++k) atomicAdd(dst + k, acc[k]);
@@ -425,9 +450,12 @@ struct PipeDepth {
 
 // W waves per workgroup (W * 64 threads), one chunk per wave at a time.
 template <int N, int V, int G, typename FT, int W>
-__global__ void __launch_bounds__(W * 64) hicoo_tile_kernel(HiCOOArgs h, FactorArgs a) {
+__global__ void __launch_bounds__(W * 64, 6) hicoo_tile_kernel(HiCOOArgs h, FactorArgs a) {
+    constexpr int NG = 64 / G;
+    constexpr int D = PipeDepth<N, V>::value;
+    constexpr int E = hstage_entries<NG, D>();
     extern __shared__ float tile[];  // (1 << sb_bits) rows x TP floats
-    __shared__ HStage<N> st_all[W];
+    __shared__ HStage<N, E> st_all[W];
     constexpr int GV = G * V;
     constexpr int TP = GV + 1;
     constexpr int NT = W * 64;
@@ -438,8 +466,8 @@ __global__ void __launch_bounds__(W * 64) hicoo_tile_kernel(HiCOOArgs h, FactorA
     __syncthreads();
     const uint32_t c0 = h.unit_chunk[u], c1 = h.unit_chunk[u + 1];
     for (uint32_t c = c0 + wid; c < c1; c += W) {
-        const uint32_t cnt = stage_hicoo<N, V>(h, a, c, st_all[wid], lane);
-        if (cnt) hicoo_slices<N, V, G, PipeDepth<N, V>::value, FT>(a, st_all[wid], cnt, tile, TP, lane);
+        const uint32_t cnt = stage_hicoo<N, V, NG, D, E>(h, a, c, st_all[wid], lane);
+        if (cnt) hicoo_slices<N, V, G, D, FT, E>(a, st_all[wid], cnt, tile, TP, lane);
     }
     __syncthreads();
     const uint32_t base = h.unit_br[u] << h.sb_bits;
@@ -459,9 +487,9 @@ __global__ void __launch_bounds__(W * 64) hicoo_tile_kernel(HiCOOArgs h, FactorA
 }
 
 // Static LDS of the HiCOO kernel's staging area for W waves.
-template <int N>
+template <int N, int V, int G>
 constexpr size_t hicoo_stage_bytes(int W) {
-    return (size_t)W * sizeof(HStage<N>);
+    return (size_t)W * sizeof(HStage<N, hstage_entries<64 / G, PipeDepth<N, V>::value>()>);
 }
 
 template <int N, int V, int G, typename FT>
