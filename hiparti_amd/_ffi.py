@@ -104,6 +104,7 @@ SIGNATURES = {
     "ptiCopySparseTensor": (C.c_int, [_P(ptiSparseTensor), _P(ptiSparseTensor), C.c_int]),
     "ptiFreeSparseTensor": (None, [_P(ptiSparseTensor)]),
     "ptiLoadSparseTensor": (C.c_int, [_P(ptiSparseTensor), ptiIndex, C.c_char_p]),
+    "ptiDumpSparseTensor": (C.c_int, [_P(ptiSparseTensor), ptiIndex, _vp]),
     "ptiSparseTensorSortIndex": (None, [_P(ptiSparseTensor), C.c_int, C.c_int]),
     "SparseTensorFrobeniusNormSquared": (C.c_double, [_P(ptiSparseTensor)]),
     "ptiResizeIndexVector": (C.c_int, [_P(ptiIndexVector), ptiNnzIndex]),

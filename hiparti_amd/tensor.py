@@ -86,6 +86,40 @@ def _mats_array(mats):
     return arr
 
 
+class _CFile:
+    """A C FILE* for the dump functions (libc fopen / fclose)."""
+    _libc = None
+
+    def __init__(self, path, mode):
+        if _CFile._libc is None:
+            libc = C.CDLL(None)
+            libc.fopen.restype = C.c_void_p
+            libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+            libc.fclose.argtypes = [C.c_void_p]
+            _CFile._libc = libc
+        self.path, self.mode = path, mode
+
+    def __enter__(self):
+        self.fp = _CFile._libc.fopen(self.path.encode(), self.mode.encode())
+        if not self.fp:
+            raise OSError("cannot open %s" % self.path)
+        return C.c_void_p(self.fp)
+
+    def __exit__(self, *exc):
+        _CFile._libc.fclose(C.c_void_p(self.fp))
+        return False
+
+
+def _kruskal_out(k, N, R):
+    lam = np.array(k.lambda_[:R], np.float32)
+    facs = []
+    for m in range(N):
+        f = k.factors[m].contents
+        a = np.ctypeslib.as_array(f.values, shape=(int(f.nrows), int(f.stride)))[:, :R].copy()
+        facs.append(a)
+    return lam, facs, float(k.fit)
+
+
 def _default_order(mode, nmodes):
     return [mode] + [(mode + i) % nmodes for i in range(1, nmodes)]
 
@@ -140,6 +174,11 @@ class COOTensor:
         self._owned = True
         return self
 
+    def dump(self, filename, start_index=1):
+        """Text coordinate dump (ptiDumpSparseTensor, HiParTIPy/Tensor.py:21-23)."""
+        with _CFile(filename, "w") as fp:
+            check(lib().ptiDumpSparseTensor(C.byref(self.address), int(start_index), fp), "ptiDumpSparseTensor")
+
     # ---- accessors --------------------------------------------------------
     def nmodes(self):
         return int(self.address.nmodes)
@@ -182,9 +221,11 @@ class COOTensor:
         h.max_nnzb = int(mx.value)
         return h
 
-    def MTTKRP(self, mode, mats=None, matsorder=None):
+    def MTTKRP(self, mode, mats=None, matsorder=None, type="default", testing=False):
         """GPU MTTKRP through ptiMTTKRP.  mats: nmodes+1 DenseMatrix objects,
-        the last one receives the result; returns it as a numpy array."""
+        the last one receives the result; returns it as a numpy array.
+        `type` selects among the reference's CPU/CUDA variants in HiParTIPy
+        (Tensor.py:128-146); every variant here is the same GPU kernel."""
         N = self.nmodes()
         if matsorder is None:
             matsorder = _default_order(mode, N)
@@ -195,6 +236,19 @@ class COOTensor:
 
     def norm_squared(self):
         return float(lib().SparseTensorFrobeniusNormSquared(C.byref(self.address)))
+
+    def cpd(self, rank, niters=50, tol=1e-5):
+        """ptiCpdAls (reference src/sptensor/cpd.c:163-208) on the GPU:
+        returns (lambda, factors, fit) as numpy arrays and a float."""
+        k = F.ptiKruskalTensor()
+        check(lib().ptiNewKruskalTensor(C.byref(k), self.nmodes(), _as_index_array(self.ndims()), int(rank)),
+              "ptiNewKruskalTensor")
+        try:
+            check(lib().ptiCpdAls(C.byref(self.address), int(rank), int(niters), float(tol), C.byref(k)),
+                  "ptiCpdAls")
+            return _kruskal_out(k, self.nmodes(), int(rank))
+        finally:
+            lib().ptiFreeKruskalTensor(C.byref(k))
 
     def free(self):
         if self._owned:
@@ -232,7 +286,21 @@ class HiCOOTensor:
         """Copies of the HiCOO arrays (see hicoo_struct_arrays)."""
         return hicoo_struct_arrays(self.address)
 
-    def MTTKRP(self, mode, mats=None, matsorder=None):
+    def cpd(self, rank, niters=50, tol=1e-5):
+        """ptiCpdAlsHiCOO (reference src/sptensor/hicoo/cpd.c:153-204) on the
+        GPU, rank <= 248: returns (lambda, factors, fit)."""
+        k = F.ptiRankKruskalTensor()
+        nd = [self.dim_mode(m) for m in range(self.nmodes())]
+        check(lib().ptiNewRankKruskalTensor(C.byref(k), self.nmodes(), _as_index_array(nd), int(rank)),
+              "ptiNewRankKruskalTensor")
+        try:
+            check(lib().ptiCpdAlsHiCOO(C.byref(self.address), int(rank), int(niters), float(tol), C.byref(k)),
+                  "ptiCpdAlsHiCOO")
+            return _kruskal_out(k, self.nmodes(), int(rank))
+        finally:
+            lib().ptiFreeRankKruskalTensor(C.byref(k))
+
+    def MTTKRP(self, mode, mats=None, matsorder=None, type="default", testing=False):
         N = self.nmodes()
         if matsorder is None:
             matsorder = _default_order(mode, N)
