@@ -154,7 +154,7 @@ def cpd_leg(args):
     t0 = time.time()
     X = COOTensor.synthetic(ndims, nnz, dist=[1, 1, 1, 0, 0], zipf_s=[1.1, 1.1, 1.1, 0, 0],
                             permute=[0, 0, 0, 0, 0], seed=args.seed + 4, nthreads=nthreads)
-    H = X.toHiCOO(7, 14, 14, nthreads=nthreads)
+    H = X.toHiCOO(7, 14, 14, nthreads=nthreads, device=True)
     X.free()
     D = DevHiCOO(H)
     nb = H.nblocks()
@@ -209,9 +209,9 @@ def main():
                             nthreads=nthreads)
     log("generated %d nnz in %.1f s" % (X.nnz(), time.time() - t0))
     t0 = time.time()
-    H = X.toHiCOO(7, 14, 14, nthreads=nthreads)
+    H = X.toHiCOO(7, 14, 14, nthreads=nthreads, device=True)
     nb = H.nblocks()
-    log("HiCOO conversion: nb=%d in %.1f s" % (nb, time.time() - t0))
+    log("HiCOO conversion (GPU ordering): nb=%d in %.1f s" % (nb, time.time() - t0))
     X.free()
     t0 = time.time()
     D = DevHiCOO(H)

@@ -112,6 +112,8 @@ SIGNATURES = {
     # HiCOO
     "ptiSparseTensorToHiCOO": (C.c_int, [_P(ptiSparseTensorHiCOO), _P(ptiNnzIndex), _P(ptiSparseTensor),
                                          ptiElementIndex, ptiElementIndex, ptiElementIndex, C.c_int]),
+    "ptiDevSparseTensorToHiCOO": (C.c_int, [_P(ptiSparseTensorHiCOO), _P(ptiNnzIndex), _P(ptiSparseTensor),
+                                            ptiElementIndex, ptiElementIndex, ptiElementIndex, _vp]),
     "ptiFreeSparseTensorHiCOO": (None, [_P(ptiSparseTensorHiCOO)]),
     # MTTKRP
     "ptiMTTKRP": (C.c_int, [_P(ptiSparseTensor), _P(_P(ptiMatrix)), _P(ptiIndex), ptiIndex]),

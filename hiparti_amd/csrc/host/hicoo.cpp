@@ -22,14 +22,18 @@
 // sort instead of swap by swap; the final COO order is the same.
 #include "internal.h"
 #include "keys.h"
+#include "hicoo_build.h"
 
+#include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 
 namespace {
 
+using pti::CompositeLayout;
 using pti::u128;
 
 int new_vec_array(ptiIndexVector **out, ptiIndex n) {
@@ -81,36 +85,6 @@ inline u128 morton4_key(uint32_t x, uint32_t y, uint32_t z, uint32_t w) {
                               ((w >> i) & 1u));
     }
     return k;
-}
-
-// Composite (superblock tuple, compressed Morton) key for the unique-order
-// fast path.  Bits of mode m below lb[m] take part in the Morton interleave;
-// bits from sk upward form the superblock tuple.  Returns false if the key
-// needs more than 128 bits.
-struct CompositeLayout {
-    int N = 0, sk = 0;
-    std::vector<int> lb, hb;
-    int morton_bits = 0, total_bits = 0, max_lb = 0;
-};
-
-bool composite_layout(const std::vector<uint32_t> &mx, int sk, CompositeLayout &L) {
-    L.N = (int)mx.size();
-    L.sk = sk;
-    L.lb.resize(L.N);
-    L.hb.resize(L.N);
-    L.morton_bits = 0;
-    L.max_lb = 0;
-    int hbits = 0;
-    for (int m = 0; m < L.N; ++m) {
-        int b = pti::bit_width_u32(mx[m]);
-        L.lb[m] = std::min(sk, b);
-        L.hb[m] = std::max(0, b - sk);
-        L.morton_bits += L.lb[m];
-        hbits += L.hb[m];
-        L.max_lb = std::max(L.max_lb, L.lb[m]);
-    }
-    L.total_bits = L.morton_bits + hbits;
-    return L.total_bits <= 128;
 }
 
 template <typename K>
@@ -180,6 +154,65 @@ int set_kernel_pointers(ptiNnzIndexVector *kptr, ptiNnzIndexVector *knnzs, const
     PTI_CHECK(ptiAppendNnzIndexVector(kptr, nnz), "HiSpTns Convert");
     PTI_CHECK(ptiAppendNnzIndexVector(knnzs, knnz), "HiSpTns Convert");
     return 0;
+}
+
+// Exact Morton ordering inside each superblock for tensors whose order is not
+// unique; superblocks are independent and are processed in parallel.
+void exact_morton_within_superblocks(ptiSparseTensor *tsr, const ptiNnzIndexVector &kptr, int nthreads) {
+    const ptiIndex N = tsr->nmodes;
+    const uint64_t nnz = tsr->nnz;
+    std::vector<u128> key(nnz);
+    std::vector<uint32_t> id(nnz);
+    pti::parallel_for(nnz, nthreads, [&](uint64_t b, uint64_t e, int) {
+        for (uint64_t z = b; z < e; ++z) {
+            const uint32_t x = tsr->inds[0].data[z], y = tsr->inds[1].data[z], zz = tsr->inds[2].data[z];
+            key[z] = N == 3 ? morton3_key(x, y, zz) : morton4_key(x, y, zz, tsr->inds[3].data[z]);
+            id[z] = (uint32_t)z;
+        }
+    });
+    const uint64_t nk = kptr.len - 1;
+    std::atomic<uint64_t> next(0);
+    int nt = nthreads > 0 ? nthreads : pti::default_threads();
+    std::vector<std::thread> ts;
+    for (int t = 0; t < nt; ++t)
+        ts.emplace_back([&] {
+            for (;;) {
+                uint64_t k = next.fetch_add(1);
+                if (k >= nk) break;
+                uint64_t b = kptr.data[k], e = kptr.data[k + 1];
+                if (e - b >= 2) pti::hoare_sort_pairs<u128>(key.data(), id.data(), b, e, 1);
+            }
+        });
+    for (auto &th : ts) th.join();
+    pti::apply_permutation(tsr, id, nthreads);
+}
+
+}  // namespace
+
+namespace pti {
+
+// Composite (superblock tuple, compressed Morton) key for the unique-order
+// fast path.  Bits of mode m below lb[m] take part in the Morton interleave;
+// bits from sk upward form the superblock tuple.  Returns false if the key
+// needs more than 128 bits.
+bool composite_layout(const std::vector<uint32_t> &mx, int sk, CompositeLayout &L) {
+    L.N = (int)mx.size();
+    L.sk = sk;
+    L.lb.resize(L.N);
+    L.hb.resize(L.N);
+    L.morton_bits = 0;
+    L.max_lb = 0;
+    int hbits = 0;
+    for (int m = 0; m < L.N; ++m) {
+        int b = pti::bit_width_u32(mx[m]);
+        L.lb[m] = std::min(sk, b);
+        L.hb[m] = std::max(0, b - sk);
+        L.morton_bits += L.lb[m];
+        hbits += L.hb[m];
+        L.max_lb = std::max(L.max_lb, L.lb[m]);
+    }
+    L.total_bits = L.morton_bits + hbits;
+    return L.total_bits <= 128;
 }
 
 inline uint32_t coord_or_zero(const ptiSparseTensor *tsr, ptiIndex m, uint64_t z) {
@@ -283,38 +316,47 @@ int set_balanced_schedule(ptiSparseTensorHiCOO *h, int sk) {
     return 0;
 }
 
-// Exact Morton ordering inside each superblock for tensors whose order is not
-// unique; superblocks are independent and are processed in parallel.
-void exact_morton_within_superblocks(ptiSparseTensor *tsr, const ptiNnzIndexVector &kptr, int nthreads) {
-    const ptiIndex N = tsr->nmodes;
-    const uint64_t nnz = tsr->nnz;
-    std::vector<u128> key(nnz);
-    std::vector<uint32_t> id(nnz);
-    pti::parallel_for(nnz, nthreads, [&](uint64_t b, uint64_t e, int) {
-        for (uint64_t z = b; z < e; ++z) {
-            const uint32_t x = tsr->inds[0].data[z], y = tsr->inds[1].data[z], zz = tsr->inds[2].data[z];
-            key[z] = N == 3 ? morton3_key(x, y, zz) : morton4_key(x, y, zz, tsr->inds[3].data[z]);
-            id[z] = (uint32_t)z;
-        }
-    });
-    const uint64_t nk = kptr.len - 1;
-    std::atomic<uint64_t> next(0);
-    int nt = nthreads > 0 ? nthreads : pti::default_threads();
-    std::vector<std::thread> ts;
-    for (int t = 0; t < nt; ++t)
-        ts.emplace_back([&] {
-            for (;;) {
-                uint64_t k = next.fetch_add(1);
-                if (k >= nk) break;
-                uint64_t b = kptr.data[k], e = kptr.data[k + 1];
-                if (e - b >= 2) pti::hoare_sort_pairs<u128>(key.data(), id.data(), b, e, 1);
+// Chunk pointers and block-offset superblock pointers, per block
+// (reference: convert.c:728-835).  On entry h->bptr holds the block starts
+// and h->kptr the superblock starts as nonzero offsets; on return kptr holds
+// block offsets and cptr the chunk boundaries.  The reference walks the
+// nonzeros; at the start of block b >= 1 its counter of the open block equals
+// the size of block b-1, so the same rule is evaluated once per block: a new
+// chunk starts at block b when the finished blocks of the current chunk plus
+// block b-1 reach 2^sc_bits nonzeros, and every superblock starts a chunk.
+int set_chunks_from_blocks(ptiSparseTensorHiCOO *h) {
+    const uint64_t nb = h->bptr.len - 1;
+    const uint64_t nk = h->kptr.len - 1;
+    const uint64_t sc = 1ull << h->sc_bits;
+    const uint64_t *bp = h->bptr.data;
+    // First block of each superblock (superblock starts are block starts).
+    std::vector<uint64_t> fb(nk + 1);
+    for (uint64_t k = 0; k < nk; ++k) {
+        const uint64_t z = h->kptr.data[k];
+        fb[k] = (uint64_t)(std::lower_bound(bp, bp + nb, z) - bp);
+    }
+    fb[nk] = nb;
+    for (uint64_t k = 0; k < nk; ++k) {
+        h->kptr.data[k] = k == 0 ? 0 : fb[k];
+        PTI_CHECK(ptiAppendNnzIndexVector(&h->cptr, h->kptr.data[k]), "HiSpTns Convert");
+        uint64_t chunk_size = 0;
+        for (uint64_t b = fb[k]; b < fb[k + 1]; ++b) {
+            if (b == 0) continue;
+            const uint64_t prev = bp[b] - bp[b - 1];
+            if (chunk_size + prev >= sc) {
+                PTI_CHECK(ptiAppendNnzIndexVector(&h->cptr, b), "HiSpTns Convert");
+                chunk_size = 0;
+            } else {
+                chunk_size += prev;
             }
-        });
-    for (auto &th : ts) th.join();
-    pti::apply_permutation(tsr, id, nthreads);
+        }
+    }
+    h->kptr.data[nk] = nb;
+    PTI_CHECK(ptiAppendNnzIndexVector(&h->cptr, nb), "HiSpTns Convert");
+    return 0;
 }
 
-}  // namespace
+}  // namespace pti
 
 extern "C" {
 
@@ -425,6 +467,13 @@ int ptiSparseTensorToHiCOO(ptiSparseTensorHiCOO *hitsr, ptiNnzIndex *max_nnzb, p
     if (N == 0 || N > 8) PTI_FAIL(PTIERR_VALUE_ERROR, "HiSpTns Convert", "nmodes must be in [1, 8]");
     if (nnz >= (1ull << 32)) PTI_FAIL(PTIERR_VALUE_ERROR, "HiSpTns Convert", "nnz must be < 2^32");
     const int nthreads = tk > 0 ? tk : pti::default_threads();
+    auto t_start = std::chrono::steady_clock::now();
+    auto phase = [&](const char *what) {
+        if (!pti::verbose()) return;
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[HiSpTns Convert] %s: %.3f s\n", what, std::chrono::duration<double>(now - t_start).count());
+        t_start = now;
+    };
 
     PTI_CHECK(ptiNewSparseTensorHiCOO(hitsr, N, tsr->ndims, nnz, sb_bits, sk_bits, sc_bits), "HiSpTns Convert");
     for (ptiIndex m = 0; m < N; ++m) tsr->sortorder[m] = m;
@@ -437,7 +486,7 @@ int ptiSparseTensorToHiCOO(ptiSparseTensorHiCOO *hitsr, ptiNnzIndex *max_nnzb, p
         if (N == 3)
             for (uint32_t v : mx) quirk_free = quirk_free && v < (1u << 24);
         CompositeLayout L;
-        if (quirk_free && composite_layout(mx, sk_bits, L)) {
+        if (quirk_free && pti::composite_layout(mx, sk_bits, L)) {
             std::vector<uint32_t> perm;
             bool unique = L.total_bits <= 64 ? try_unique_order<uint64_t>(tsr, L, perm, nthreads)
                                              : try_unique_order<u128>(tsr, L, perm, nthreads);
@@ -452,15 +501,16 @@ int ptiSparseTensorToHiCOO(ptiSparseTensorHiCOO *hitsr, ptiNnzIndex *max_nnzb, p
         pti::apply_permutation(tsr, perm, nthreads);
     }
 
+    phase(ordered ? "order (composite-key radix sort)" : "order (exact quicksort replay)");
     // ---- 2. superblocks and schedules --------------------------------------
     PTI_CHECK(set_kernel_pointers(&hitsr->kptr, &hitsr->knnzs, tsr, sk_bits), "HiSpTns Convert");
-    PTI_CHECK(set_kernel_scheduler(hitsr, tsr, sk_bits), "HiSpTns Convert");
-    PTI_CHECK(set_balanced_schedule(hitsr, sk_bits), "HiSpTns Convert");
+    PTI_CHECK(pti::set_kernel_scheduler(hitsr, tsr, sk_bits), "HiSpTns Convert");
+    PTI_CHECK(pti::set_balanced_schedule(hitsr, sk_bits), "HiSpTns Convert");
     if (!ordered && (N == 3 || N == 4) && nnz >= 2) exact_morton_within_superblocks(tsr, hitsr->kptr, nthreads);
 
+    phase("superblocks, schedules");
     // ---- 3. blocks, element offsets, chunks --------------------------------
     const uint32_t emask = (1u << sb_bits) - 1u;
-    const uint64_t sc = 1ull << sc_bits;
     std::vector<uint8_t> is_new(nnz, 0);
     pti::parallel_for(nnz, nthreads, [&](uint64_t b, uint64_t e, int) {
         for (uint64_t z = b; z < e; ++z) {
@@ -491,7 +541,7 @@ int ptiSparseTensorToHiCOO(ptiSparseTensorHiCOO *hitsr, ptiNnzIndex *max_nnzb, p
     }
     // Block starts and coordinates.
     hitsr->bptr.data[0] = 0;
-    for (ptiIndex m = 0; m < N; ++m) hitsr->binds[m].data[0] = coord_or_zero(tsr, m, 0) >> sb_bits;
+    for (ptiIndex m = 0; m < N; ++m) hitsr->binds[m].data[0] = pti::coord_or_zero(tsr, m, 0) >> sb_bits;
     {
         uint64_t b = 0;
         for (uint64_t z = 1; z < nnz; ++z) {
@@ -503,42 +553,9 @@ int ptiSparseTensorToHiCOO(ptiSparseTensorHiCOO *hitsr, ptiNnzIndex *max_nnzb, p
     }
     hitsr->bptr.data[nb] = nnz;
 
-    // Superblock pointers become block offsets; chunk pointers are cut when
-    // the running nnz count of the finished blocks reaches 2^sc_bits, and a
-    // chunk always starts at a superblock (reference: convert.c:728-835).
-    {
-        const uint64_t nk = hitsr->kptr.len - 1;
-        uint64_t blk = 0;          // index of the block holding the current nnz
-        uint64_t ne = 0;           // nnz of the currently open block
-        uint64_t chunk_size = 0;   // nnz in the current chunk, closed blocks only
-        uint64_t nb_count = 1;     // blocks opened so far
-        for (uint64_t k = 0; k < nk; ++k) {
-            uint64_t kb = hitsr->kptr.data[k], ke = hitsr->kptr.data[k + 1];
-            uint64_t nb_tmp = k == 0 ? 0 : nb_count;
-            hitsr->kptr.data[k] = nb_tmp;
-            PTI_CHECK(ptiAppendNnzIndexVector(&hitsr->cptr, nb_tmp), "HiSpTns Convert");
-            chunk_size = 0;
-            for (uint64_t z = kb; z < ke; ++z) {
-                if (z == 0 || !is_new[z]) {
-                    ++ne;
-                } else {
-                    if (chunk_size + ne >= sc) {
-                        PTI_CHECK(ptiAppendNnzIndexVector(&hitsr->cptr, nb_count), "HiSpTns Convert");
-                        chunk_size = 0;
-                    } else {
-                        chunk_size += ne;
-                    }
-                    ++nb_count;
-                    ++blk;
-                    ne = 1;
-                }
-            }
-        }
-        (void)blk;
-        hitsr->kptr.data[hitsr->kptr.len - 1] = nb;
-        PTI_CHECK(ptiAppendNnzIndexVector(&hitsr->cptr, nb), "HiSpTns Convert");
-    }
+    PTI_CHECK(pti::set_chunks_from_blocks(hitsr), "HiSpTns Convert");
 
+    phase("blocks, element offsets, chunks");
     uint64_t mx_nnzb = hitsr->bptr.data[1] - hitsr->bptr.data[0];
     for (uint64_t b = 0; b < nb; ++b) mx_nnzb = std::max(mx_nnzb, hitsr->bptr.data[b + 1] - hitsr->bptr.data[b]);
     if (max_nnzb) *max_nnzb = mx_nnzb;

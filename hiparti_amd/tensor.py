@@ -210,13 +210,19 @@ class COOTensor:
     def sort(self, nthreads=0):
         lib().ptiSparseTensorSortIndex(C.byref(self.address), 1, int(nthreads))
 
-    def toHiCOO(self, b=7, k=7, c=7, nthreads=0):
+    def toHiCOO(self, b=7, k=7, c=7, nthreads=0, device=False):
         """Convert to HiCOO.  As in the reference, this sorts the COO tensor
-        in place (src/sptensor/hicoo/convert.c:454,590)."""
+        in place (src/sptensor/hicoo/convert.c:454,590).  device=True orders
+        and emits the blocks on the GPU (ptiDevSparseTensorToHiCOO); the
+        result is the same."""
         h = HiCOOTensor()
         mx = F.ptiNnzIndex()
-        check(lib().ptiSparseTensorToHiCOO(C.byref(h.address), C.byref(mx), C.byref(self.address),
-                                           int(b), int(k), int(c), int(nthreads)), "ptiSparseTensorToHiCOO")
+        if device:
+            check(lib().ptiDevSparseTensorToHiCOO(C.byref(h.address), C.byref(mx), C.byref(self.address),
+                                                  int(b), int(k), int(c), None), "ptiDevSparseTensorToHiCOO")
+        else:
+            check(lib().ptiSparseTensorToHiCOO(C.byref(h.address), C.byref(mx), C.byref(self.address),
+                                               int(b), int(k), int(c), int(nthreads)), "ptiSparseTensorToHiCOO")
         h._owned = True
         h.max_nnzb = int(mx.value)
         return h

@@ -114,6 +114,17 @@ int ptiDevCpdAlsCOO(ptiDevCOO *t, float * const dfacs[], ptiIndex R, ptiIndex st
                     double tol, double *dlambda, double *fit, double *fit_trace, ptiIndex *iters_done,
                     void *stream);
 
+/* ptiSparseTensorToHiCOO with the ordering and the block emission on the
+ * GPU (the reference converts on the host, src/sptensor/hicoo/convert.c:
+ * 646-860).  Same arguments, same output bit for bit, and like the
+ * reference it sorts the caller's COO tensor in place.  The GPU path covers
+ * tensors whose reference order is unique (3-D / 4-D, distinct coordinates,
+ * 3-D indices < 2^24, a composite key of <= 64 bits); every other tensor is
+ * converted by the host code of ptiSparseTensorToHiCOO. */
+int ptiDevSparseTensorToHiCOO(ptiSparseTensorHiCOO *hitsr, ptiNnzIndex *max_nnzb, ptiSparseTensor *tsr,
+                              ptiElementIndex const sb_bits, ptiElementIndex const sk_bits,
+                              ptiElementIndex const sc_bits, void *stream);
+
 /* Register a host HiCOO / COO tensor so that the drop-in ptiMTTKRPHiCOO /
  * ptiMTTKRP reuse a device copy instead of uploading the tensor per call.
  * The caller must unregister before modifying or freeing the host tensor. */
