@@ -203,6 +203,8 @@ int gpu_convert(ptiSparseTensorHiCOO *hitsr, ptiNnzIndex *max_nnzb, ptiSparseTen
     hipLaunchKernelGGL(gather_sorted_kernel, grid_for(nnz), dim3(256), 0, s, in, dvals.as<float>(),
                        ids1.as<uint32_t>(), nnz, N, sorted, dsv.as<float>());
     PTI_HIP(hipGetLastError(), M);
+    // The gather must finish before its inputs are released.
+    PTI_HIP(hipStreamSynchronize(s), M);
     for (uint32_t m = 0; m < N; ++m) din[m].release();
     dvals.release();
     ids1.release();
@@ -217,7 +219,7 @@ int gpu_convert(ptiSparseTensorHiCOO *hitsr, ptiNnzIndex *max_nnzb, ptiSparseTen
     PTI_HIP(rocprim::inclusive_scan(nullptr, sbytes, bflag.as<uint32_t>(), bid.as<uint32_t>(), (size_t)nnz,
                                     rocprim::plus<uint32_t>(), s),
             M);
-    PTI_HIP(tmp.alloc(sbytes), M);
+    PTI_HIP(tmp.alloc(sbytes), M);   // the sort's storage is idle: the stream was synchronised
     PTI_HIP(rocprim::inclusive_scan(tmp.ptr, sbytes, bflag.as<uint32_t>(), bid.as<uint32_t>(), (size_t)nnz,
                                     rocprim::plus<uint32_t>(), s),
             M);

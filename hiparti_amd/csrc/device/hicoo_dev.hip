@@ -28,12 +28,18 @@
 #include <unordered_map>
 
 #include "../../../include/hiparti_dev.h"
-#include "../host/sorting.h"
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
 #include "dropin.hpp"
 #include "launch.hpp"
 
 namespace {
 constexpr uint32_t UNIT_CHUNKS = 64;  // chunks (of 256 nonzeros) per workgroup unit
+
+struct DimArgs {
+    uint32_t d[8];
+};
 }
 
 struct ptiDevHiCOOMode {
@@ -78,7 +84,224 @@ namespace {
 std::mutex g_reg_mu;
 std::unordered_map<const void *, ptiDevHiCOO *> g_hicoo_reg;
 
-int validate(const ptiSparseTensorHiCOO *hitsr, const char *M) {
+// The host tensor's arrays in device memory, shared by the per-mode builds.
+struct RawHiCOO {
+    Buffer bptr, vals, binds[8], einds[8];
+};
+
+struct Ptr8u32 {
+    const uint32_t *p[8];
+};
+struct Ptr8u8 {
+    const uint8_t *p[8];
+};
+
+inline dim3 grid_n(uint64_t n) {
+    uint64_t b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    if (b == 0) b = 1;
+    return dim3((uint32_t)b);
+}
+
+// Every block must be non-empty and inside [0, nnz), every element offset
+// below 2^sb_bits and every coordinate inside its mode: the kernels use them
+// as memory and LDS indices.
+__global__ void __launch_bounds__(256) validate_blocks_kernel(const uint64_t *__restrict__ bptr, uint64_t nb,
+                                                              uint64_t nnz, Ptr8u32 binds, Ptr8u8 einds, uint32_t N,
+                                                              uint32_t sb, DimArgs dims, int *bad) {
+    for (uint64_t b = (uint64_t)blockIdx.x * 256 + threadIdx.x; b < nb; b += (uint64_t)gridDim.x * 256) {
+        const uint64_t z0 = bptr[b], z1 = bptr[b + 1];
+        if (z1 <= z0 || z1 > nnz) {
+            atomicOr(bad, 1);
+            continue;
+        }
+        for (uint32_t m = 0; m < N; ++m) {
+            const uint64_t base = (uint64_t)binds.p[m][b] << sb;
+            uint32_t emax = 0;
+            for (uint64_t z = z0; z < z1; ++z) emax = max(emax, (uint32_t)einds.p[m][z]);
+            if (emax >> sb || base + emax >= dims.d[m]) atomicOr(bad, 1);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) iota_u32_kernel(uint32_t *p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+        p[i] = (uint32_t)i;
+}
+
+__global__ void __launch_bounds__(256) max_u32_kernel(const uint32_t *p, uint64_t n, uint32_t *out) {
+    uint32_t v = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+        v = max(v, p[i]);
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, v);
+}
+
+// Per reordered block i: its size, its coordinates, and whether it starts a
+// block-row (a new mode-m block coordinate).
+__global__ void __launch_bounds__(256) block_info_kernel(const uint64_t *__restrict__ bptr,
+                                                         const uint32_t *__restrict__ skey,
+                                                         const uint32_t *__restrict__ sid, uint64_t nb, Ptr8u32 binds,
+                                                         uint32_t N, uint32_t *__restrict__ bsz,
+                                                         uint32_t *__restrict__ hbi, uint32_t *__restrict__ segflag) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nb; i += (uint64_t)gridDim.x * 256) {
+        const uint32_t b = sid[i];
+        bsz[i] = (uint32_t)(bptr[b + 1] - bptr[b]);
+        for (uint32_t m = 0; m < N; ++m) hbi[i * N + m] = binds.p[m][b];
+        segflag[i] = (i == 0 || skey[i] != skey[i - 1]) ? 1u : 0u;
+    }
+}
+
+// seg_first[s] = first reordered block of block-row s.
+__global__ void __launch_bounds__(256) segment_starts_kernel(const uint32_t *__restrict__ segflag,
+                                                             const uint32_t *__restrict__ segid, uint64_t nb,
+                                                             uint32_t *__restrict__ seg_first) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nb; i += (uint64_t)gridDim.x * 256)
+        if (segflag[i]) seg_first[segid[i] - 1] = (uint32_t)i;
+}
+
+// Chunks and units per block-row: nch = ceil(nnz / CHUNK), nu = ceil(nch / UNIT_CHUNKS).
+__global__ void __launch_bounds__(256) segment_counts_kernel(const uint32_t *__restrict__ seg_first,
+                                                             const uint32_t *__restrict__ hb, uint32_t nseg,
+                                                             uint32_t unit_chunks, uint32_t *__restrict__ nch,
+                                                             uint32_t *__restrict__ nu) {
+    for (uint32_t s = blockIdx.x * 256 + threadIdx.x; s < nseg; s += gridDim.x * 256) {
+        const uint32_t len = hb[seg_first[s + 1]] - hb[seg_first[s]];
+        const uint32_t c = (len + CHUNK - 1) / CHUNK;
+        nch[s] = c;
+        nu[s] = (c + unit_chunks - 1) / unit_chunks;
+    }
+}
+
+__device__ inline uint32_t upper_bound_u32(const uint32_t *a, uint32_t lo, uint32_t hi, uint32_t v) {
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (a[mid] <= v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) chunk_fill_kernel(const uint32_t *__restrict__ choff,
+                                                         const uint32_t *__restrict__ seg_first,
+                                                         const uint32_t *__restrict__ hb, uint32_t nseg,
+                                                         uint32_t nchunks, uint32_t *__restrict__ chunk_z,
+                                                         uint32_t *__restrict__ chunk_blk) {
+    for (uint32_t c = blockIdx.x * 256 + threadIdx.x; c < nchunks; c += gridDim.x * 256) {
+        const uint32_t s = upper_bound_u32(choff, 0, nseg + 1, c) - 1;
+        const uint32_t first = seg_first[s], next = seg_first[s + 1];
+        const uint32_t z = hb[first] + (c - choff[s]) * CHUNK;
+        chunk_z[c] = z;
+        chunk_blk[c] = upper_bound_u32(hb, first, next, z) - 1;
+    }
+}
+
+__global__ void __launch_bounds__(256) unit_fill_kernel(const uint32_t *__restrict__ uoff,
+                                                        const uint32_t *__restrict__ choff,
+                                                        const uint32_t *__restrict__ seg_first,
+                                                        const uint32_t *__restrict__ skey, uint32_t nseg,
+                                                        uint32_t nunits, uint32_t *__restrict__ unit_chunk,
+                                                        uint32_t *__restrict__ unit_br,
+                                                        uint32_t *__restrict__ unit_excl) {
+    for (uint32_t U = blockIdx.x * 256 + threadIdx.x; U < nunits; U += gridDim.x * 256) {
+        const uint32_t s = upper_bound_u32(uoff, 0, nseg + 1, U) - 1;
+        const uint64_t u = U - uoff[s], nu = uoff[s + 1] - uoff[s], nch = choff[s + 1] - choff[s];
+        unit_chunk[U] = choff[s] + (uint32_t)(u * nch / nu);
+        unit_br[U] = skey[seg_first[s]];
+        unit_excl[U] = nu == 1 ? 1u : 0u;
+    }
+}
+
+// One wave per chunk: the chunk's nonzeros (in reordered block order) are
+// stably sorted by their mode-m element offset with a counting sort whose
+// ranks come from ballots, and written out with their packed offsets, value
+// and block tag.
+__global__ void __launch_bounds__(256) chunk_sort_kernel(const uint64_t *__restrict__ bptr,
+                                                         const uint32_t *__restrict__ sid,
+                                                         const uint32_t *__restrict__ hb, uint64_t nb,
+                                                         const uint32_t *__restrict__ chunk_z,
+                                                         const uint32_t *__restrict__ chunk_blk, uint32_t nchunks,
+                                                         Ptr8u8 einds, const float *__restrict__ vals, uint32_t N,
+                                                         uint32_t mode, uint32_t sb, uint32_t *__restrict__ he,
+                                                         float *__restrict__ hv, uint8_t *__restrict__ hl) {
+    __shared__ uint32_t cnt_all[4][129];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t c = blockIdx.x * 4 + wid;
+    if (c >= nchunks) return;   // whole wave
+    uint32_t *cnt = cnt_all[wid];
+    const uint32_t nrow = 1u << sb;
+    const uint32_t EW = (N + 3) / 4;
+    const uint32_t z0 = chunk_z[c], n = chunk_z[c + 1] - z0, cb = chunk_blk[c];
+    const uint32_t hi = (uint32_t)min<uint64_t>((uint64_t)cb + CHUNK + 1, nb + 1);
+    for (uint32_t r = lane; r <= nrow; r += 64) cnt[r] = 0;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    uint32_t key[4], src[4], bl[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t k = r * 64 + lane;
+        key[r] = 0;
+        src[r] = 0;
+        bl[r] = 0;
+        if (k < n) {
+            const uint32_t z = z0 + k;
+            const uint32_t blk = upper_bound_u32(hb, cb, hi, z) - 1;
+            src[r] = (uint32_t)(bptr[sid[blk]] + (z - hb[blk]));
+            bl[r] = blk - cb;
+            key[r] = einds.p[mode][src[r]];
+            atomicAdd(&cnt[key[r] + 1], 1u);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0)
+        for (uint32_t v = 0; v < nrow; ++v) cnt[v + 1] += cnt[v];   // cnt[v] = first slot of key v
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t k = r * 64 + lane;
+        const bool valid = k < n;
+        uint64_t same = __ballot(valid);
+        for (uint32_t bit = 0; bit < sb; ++bit) {
+            const bool one = (key[r] >> bit) & 1u;
+            const uint64_t ones = __ballot(valid && one);
+            same &= one ? ones : ~ones;
+        }
+        const uint32_t rank = (uint32_t)__popcll(same & below);
+        uint32_t dest = 0;
+        if (valid) dest = cnt[key[r]] + rank;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) cnt[key[r]] += (uint32_t)__popcll(same);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (valid) {
+            const uint64_t z = (uint64_t)z0 + dest;
+            for (uint32_t w = 0; w < EW; ++w) {
+                uint32_t word = 0;
+                for (uint32_t m = 4 * w; m < N && m < 4 * w + 4; ++m)
+                    word |= (uint32_t)einds.p[m][src[r]] << ((m & 3) * 8);
+                he[z * EW + w] = word;
+            }
+            hv[z] = vals[src[r]];
+            hl[z] = (uint8_t)bl[r];
+        }
+    }
+}
+
+template <typename T>
+int dev_put(Buffer &b, const T *src, size_t n, hipStream_t s, const char *M) {
+    PTI_HIP(b.alloc(n * sizeof(T)), M);
+    if (n) PTI_HIP(hipMemcpyAsync(b.ptr, src, n * sizeof(T), hipMemcpyHostToDevice, s), M);
+    return 0;
+}
+
+// Copies the host tensor's arrays to the device and checks them there.
+int raw_upload(RawHiCOO &r, const ptiSparseTensorHiCOO *hitsr, hipStream_t s, const char *M) {
     const uint32_t N = hitsr->nmodes;
     if (N < 2 || N > 8) PTI_FAIL(PTIERR_VALUE_ERROR, M, "nmodes must be in [2, 8]");
     if (hitsr->sb_bits > 7) PTI_FAIL(PTIERR_VALUE_ERROR, M, "sb_bits must be <= 7");
@@ -89,154 +312,173 @@ int validate(const ptiSparseTensorHiCOO *hitsr, const char *M) {
     for (uint32_t m = 0; m < N; ++m)
         if (hitsr->einds[m].len < nnz || hitsr->binds[m].len < nb)
             PTI_FAIL(PTIERR_VALUE_ERROR, M, "inconsistent HiCOO arrays");
-    if (nb && hitsr->bptr.data[nb] != nnz) PTI_FAIL(PTIERR_VALUE_ERROR, M, "bptr does not end at nnz");
-    // The kernels index factor rows, output rows and LDS staging buffers with
-    // these arrays: every block must be non-empty (a 256-nonzero chunk then
-    // spans at most 257 blocks) and every coordinate must lie inside its mode.
-    if (nnz) {
-        std::atomic<bool> bad(false);
-        pti::parallel_for(nb, 0, [&](uint64_t b, uint64_t e, int) {
-            for (uint64_t x = b; x < e && !bad.load(std::memory_order_relaxed); ++x) {
-                const uint64_t z0 = hitsr->bptr.data[x], z1 = hitsr->bptr.data[x + 1];
-                if (z1 <= z0 || z1 > nnz) {
-                    bad = true;
-                    break;
-                }
-                for (uint32_t m = 0; m < N; ++m) {
-                    const uint64_t base = (uint64_t)hitsr->binds[m].data[x] << hitsr->sb_bits;
-                    for (uint64_t z = z0; z < z1; ++z)
-                        if (base + hitsr->einds[m].data[z] >= hitsr->ndims[m]) bad = true;
-                }
-            }
-        });
-        if (bad.load()) PTI_FAIL(PTIERR_VALUE_ERROR, M, "empty block or coordinate outside its mode");
+    if (!nnz) return 0;
+    if (hitsr->bptr.data[0] != 0 || hitsr->bptr.data[nb] != nnz)
+        PTI_FAIL(PTIERR_VALUE_ERROR, M, "bptr must start at 0 and end at nnz");
+    PTI_CHECK(dev_put(r.bptr, hitsr->bptr.data, nb + 1, s, M), M);
+    PTI_CHECK(dev_put(r.vals, hitsr->values.data, nnz, s, M), M);
+    Ptr8u32 bp{};
+    Ptr8u8 ep{};
+    DimArgs dims{};
+    for (uint32_t m = 0; m < N; ++m) {
+        PTI_CHECK(dev_put(r.binds[m], hitsr->binds[m].data, nb, s, M), M);
+        PTI_CHECK(dev_put(r.einds[m], hitsr->einds[m].data, nnz, s, M), M);
+        bp.p[m] = r.binds[m].as<uint32_t>();
+        ep.p[m] = r.einds[m].as<uint8_t>();
+        dims.d[m] = hitsr->ndims[m];
     }
+    Buffer bad;
+    PTI_HIP(bad.alloc(sizeof(int)), M);
+    PTI_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(int), s), M);
+    hipLaunchKernelGGL(validate_blocks_kernel, grid_n(nb), dim3(256), 0, s, r.bptr.as<uint64_t>(), nb, nnz, bp, ep, N,
+                       (uint32_t)hitsr->sb_bits, dims, bad.as<int>());
+    PTI_HIP(hipGetLastError(), M);
+    int h_bad = 0;
+    PTI_HIP(hipMemcpyAsync(&h_bad, bad.ptr, sizeof(int), hipMemcpyDeviceToHost, s), M);
+    PTI_HIP(hipStreamSynchronize(s), M);
+    if (h_bad) PTI_FAIL(PTIERR_VALUE_ERROR, M, "empty block or coordinate outside its mode");
     return 0;
 }
 
-// Builds and uploads the copy of target mode `mode`.
-int build_mode(ptiDevHiCOO *t, const ptiSparseTensorHiCOO *hitsr, uint32_t mode, hipStream_t s, const char *M) {
+// Builds the copy of target mode `mode` on the device from the raw arrays.
+int build_mode(ptiDevHiCOO *t, const RawHiCOO &r, uint32_t mode, hipStream_t s, const char *M) {
     const uint32_t N = t->nmodes;
     const uint64_t nnz = t->nnz, nb = t->nb;
-    const int EW = (N + 3) / 4;
+    const uint32_t EW = (N + 3) / 4;
     ptiDevHiCOOMode &c = t->mc[mode];
-
-    // Stable order of the blocks by their mode-m block coordinate.
-    std::vector<uint32_t> key(nb), ids(nb);
-    uint32_t kmax = 0;
-    for (uint64_t b = 0; b < nb; ++b) {
-        key[b] = hitsr->binds[mode].data[b];
-        ids[b] = (uint32_t)b;
-        kmax = std::max(kmax, key[b]);
+    Ptr8u32 bp{};
+    Ptr8u8 ep{};
+    for (uint32_t m = 0; m < N; ++m) {
+        bp.p[m] = r.binds[m].as<uint32_t>();
+        ep.p[m] = r.einds[m].as<uint8_t>();
     }
-    int kbits = 0;
-    while (kbits < 32 && (kmax >> kbits)) ++kbits;
-    pti::radix_sort_pairs<uint32_t>(key, ids, kbits, 0);
-
-    // Reordered blocks: nonzero offsets (block order) and coordinates.
-    std::vector<uint32_t> hb(nb + 1), hbi(nb * N);
-    hb[0] = 0;
-    for (uint64_t i = 0; i < nb; ++i) {
-        const uint64_t b = ids[i];
-        hb[i + 1] = hb[i] + (uint32_t)(hitsr->bptr.data[b + 1] - hitsr->bptr.data[b]);
-    }
-    pti::parallel_for(nb, 0, [&](uint64_t lo, uint64_t hi, int) {
-        for (uint64_t i = lo; i < hi; ++i)
-            for (uint32_t m = 0; m < N; ++m) hbi[i * N + m] = hitsr->binds[m].data[ids[i]];
-    });
-
-    // Chunks (never crossing a block-row) and units.
-    std::vector<uint32_t> cz, cb, uc, ubr, uex;
-    cz.reserve(nnz / pti::dev::CHUNK + 1024);
-    cb.reserve(nnz / pti::dev::CHUNK + 1024);
-    for (uint64_t i = 0; i < nb;) {
-        const uint32_t br = key[i];
-        uint64_t j = i;
-        while (j < nb && key[j] == br) ++j;
-        const uint64_t gz1 = hb[j];
-        const uint64_t first = cz.size();
-        uint64_t blk = i;
-        for (uint64_t z = hb[i]; z < gz1; z += pti::dev::CHUNK) {
-            while (hb[blk + 1] <= z) ++blk;
-            cz.push_back((uint32_t)z);
-            cb.push_back((uint32_t)blk);
-        }
-        const uint64_t nch = cz.size() - first;
-        const uint64_t nu = (nch + UNIT_CHUNKS - 1) / UNIT_CHUNKS;
-        for (uint64_t u = 0; u < nu; ++u) {
-            uc.push_back((uint32_t)(first + u * nch / nu));
-            ubr.push_back(br);
-            uex.push_back(nu == 1 ? 1u : 0u);
-        }
-        i = j;
-    }
-    c.nchunks = (uint32_t)cz.size();
-    c.nunits = (uint32_t)ubr.size();
-    cz.push_back((uint32_t)nnz);
-    uc.push_back(c.nchunks);
-
-    // Nonzeros: per chunk, a stable counting sort on the mode-m element
-    // offset; each nonzero records its block relative to the chunk's first.
-    std::vector<uint32_t> he(nnz * EW, 0u);
-    std::vector<float> hv(nnz);
-    std::vector<uint8_t> hl(nnz);
-    const uint32_t nrow = 1u << t->sb_bits;
-    pti::parallel_for(c.nchunks, 0, [&](uint64_t lo, uint64_t hi, int) {
-        std::vector<uint32_t> src(pti::dev::CHUNK), bl(pti::dev::CHUNK), cnt(nrow + 1);
-        for (uint64_t ch = lo; ch < hi; ++ch) {
-            const uint32_t z0 = cz[ch], n = cz[ch + 1] - z0;
-            // Source nonzero and block (in block order) of each position.
-            uint64_t blk = cb[ch];
-            for (uint32_t k = 0; k < n; ++k) {
-                while (hb[blk + 1] <= z0 + k) ++blk;
-                src[k] = (uint32_t)(hitsr->bptr.data[ids[blk]] + (z0 + k - hb[blk]));
-                bl[k] = (uint32_t)(blk - cb[ch]);
-            }
-            std::fill(cnt.begin(), cnt.end(), 0u);
-            for (uint32_t k = 0; k < n; ++k) ++cnt[hitsr->einds[mode].data[src[k]] + 1];
-            for (uint32_t r = 0; r < nrow; ++r) cnt[r + 1] += cnt[r];
-            for (uint32_t k = 0; k < n; ++k) {
-                const uint32_t sz = src[k];
-                const uint64_t z = z0 + cnt[hitsr->einds[mode].data[sz]]++;
-                for (uint32_t m = 0; m < N; ++m)
-                    he[z * EW + (m >> 2)] |= (uint32_t)hitsr->einds[m].data[sz] << ((m & 3) * 8);
-                hv[z] = hitsr->values.data[sz];
-                hl[z] = (uint8_t)bl[k];
-            }
-        }
-    });
-
-    auto put = [&](Buffer &buf, const void *src, size_t bytes) -> int {
-        PTI_HIP(buf.alloc(bytes), M);
-        if (bytes) PTI_HIP(hipMemcpyAsync(buf.ptr, src, bytes, hipMemcpyHostToDevice, s), M);
-        t->bytes += bytes;
-        return 0;
-    };
-    int rc = 0;
-    if (!rc) rc = put(c.binds, hbi.data(), hbi.size() * 4);
-    if (!rc) rc = put(c.einds, he.data(), he.size() * 4);
-    if (!rc) rc = put(c.vals, hv.data(), hv.size() * 4);
-    if (!rc) rc = put(c.lblk, hl.data(), hl.size());
-    if (!rc) rc = put(c.chunk_z, cz.data(), cz.size() * 4);
-    if (!rc) rc = put(c.chunk_blk, cb.data(), cb.size() * 4);
-    if (!rc) rc = put(c.unit_chunk, uc.data(), uc.size() * 4);
-    if (!rc) rc = put(c.unit_br, ubr.data(), ubr.size() * 4);
-    if (!rc) rc = put(c.unit_excl, uex.data(), uex.size() * 4);
-    // The copies read the host vectors above: finish them before they go.
-    hipError_t e = hipStreamSynchronize(s);
-    if (!rc && e != hipSuccess) {
-        pti_ComplainError(M, hip_code(e), __FILE__, __LINE__, hipGetErrorString(e));
-        rc = hip_code(e);
-    }
-    if (!rc) c.ready = true;
-    return rc;
+    // 1. Stable order of the blocks by their mode-m block coordinate.
+    Buffer kmax, ids, skey, sid, tmp_sort, tmp_scan, tmp_seg;
+    PTI_HIP(kmax.alloc(4), M);
+    PTI_HIP(hipMemsetAsync(kmax.ptr, 0, 4, s), M);
+    hipLaunchKernelGGL(max_u32_kernel, grid_n(nb), dim3(256), 0, s, bp.p[mode], nb, kmax.as<uint32_t>());
+    PTI_HIP(hipGetLastError(), M);
+    uint32_t h_kmax = 0;
+    PTI_HIP(hipMemcpyAsync(&h_kmax, kmax.ptr, 4, hipMemcpyDeviceToHost, s), M);
+    PTI_HIP(hipStreamSynchronize(s), M);
+    unsigned kbits = 0;
+    while (kbits < 32 && (h_kmax >> kbits)) ++kbits;
+    PTI_HIP(ids.alloc(nb * 4), M);
+    PTI_HIP(skey.alloc(nb * 4), M);
+    PTI_HIP(sid.alloc(nb * 4), M);
+    hipLaunchKernelGGL(iota_u32_kernel, grid_n(nb), dim3(256), 0, s, ids.as<uint32_t>(), nb);
+    PTI_HIP(hipGetLastError(), M);
+    size_t tb = 0;
+    PTI_HIP(rocprim::radix_sort_pairs(nullptr, tb, bp.p[mode], skey.as<uint32_t>(), ids.as<uint32_t>(),
+                                      sid.as<uint32_t>(), (size_t)nb, 0, std::max(kbits, 1u), s),
+            M);
+    PTI_HIP(tmp_sort.alloc(tb), M);
+    PTI_HIP(rocprim::radix_sort_pairs(tmp_sort.ptr, tb, bp.p[mode], skey.as<uint32_t>(), ids.as<uint32_t>(),
+                                      sid.as<uint32_t>(), (size_t)nb, 0, std::max(kbits, 1u), s),
+            M);
+    // 2. Block sizes -> nonzero offsets in the new order; coordinates; block-rows.
+    Buffer bsz, hb, segflag, segid;
+    PTI_HIP(bsz.alloc(nb * 4), M);
+    PTI_HIP(hb.alloc((nb + 1) * 4), M);
+    PTI_HIP(segflag.alloc(nb * 4), M);
+    PTI_HIP(segid.alloc(nb * 4), M);
+    PTI_HIP(c.binds.alloc(nb * N * 4), M);
+    hipLaunchKernelGGL(block_info_kernel, grid_n(nb), dim3(256), 0, s, r.bptr.as<uint64_t>(), skey.as<uint32_t>(),
+                       sid.as<uint32_t>(), nb, bp, N, bsz.as<uint32_t>(), c.binds.as<uint32_t>(),
+                       segflag.as<uint32_t>());
+    PTI_HIP(hipGetLastError(), M);
+    size_t sb1 = 0, sb2 = 0;
+    PTI_HIP(rocprim::exclusive_scan(nullptr, sb1, bsz.as<uint32_t>(), hb.as<uint32_t>(), 0u, (size_t)nb,
+                                    rocprim::plus<uint32_t>(), s),
+            M);
+    PTI_HIP(rocprim::inclusive_scan(nullptr, sb2, segflag.as<uint32_t>(), segid.as<uint32_t>(), (size_t)nb,
+                                    rocprim::plus<uint32_t>(), s),
+            M);
+    PTI_HIP(tmp_scan.alloc(std::max(sb1, sb2)), M);
+    PTI_HIP(rocprim::exclusive_scan(tmp_scan.ptr, sb1, bsz.as<uint32_t>(), hb.as<uint32_t>(), 0u, (size_t)nb,
+                                    rocprim::plus<uint32_t>(), s),
+            M);
+    const uint32_t nnz32 = (uint32_t)nnz;
+    PTI_HIP(hipMemcpyAsync(hb.as<uint32_t>() + nb, &nnz32, 4, hipMemcpyHostToDevice, s), M);
+    PTI_HIP(rocprim::inclusive_scan(tmp_scan.ptr, sb2, segflag.as<uint32_t>(), segid.as<uint32_t>(), (size_t)nb,
+                                    rocprim::plus<uint32_t>(), s),
+            M);
+    uint32_t nseg = 0;
+    PTI_HIP(hipMemcpyAsync(&nseg, segid.as<uint32_t>() + nb - 1, 4, hipMemcpyDeviceToHost, s), M);
+    PTI_HIP(hipStreamSynchronize(s), M);
+    Buffer seg_first, nch, nu, choff, uoff;
+    PTI_HIP(seg_first.alloc((size_t)(nseg + 1) * 4), M);
+    const uint32_t nb32 = (uint32_t)nb;
+    PTI_HIP(hipMemcpyAsync(seg_first.as<uint32_t>() + nseg, &nb32, 4, hipMemcpyHostToDevice, s), M);
+    hipLaunchKernelGGL(segment_starts_kernel, grid_n(nb), dim3(256), 0, s, segflag.as<uint32_t>(),
+                       segid.as<uint32_t>(), nb, seg_first.as<uint32_t>());
+    PTI_HIP(hipGetLastError(), M);
+    // 3. Chunks and units per block-row.
+    PTI_HIP(nch.alloc((size_t)(nseg + 1) * 4), M);
+    PTI_HIP(nu.alloc((size_t)(nseg + 1) * 4), M);
+    PTI_HIP(choff.alloc((size_t)(nseg + 1) * 4), M);
+    PTI_HIP(uoff.alloc((size_t)(nseg + 1) * 4), M);
+    hipLaunchKernelGGL(segment_counts_kernel, grid_n(nseg), dim3(256), 0, s, seg_first.as<uint32_t>(),
+                       hb.as<uint32_t>(), nseg, UNIT_CHUNKS, nch.as<uint32_t>(), nu.as<uint32_t>());
+    PTI_HIP(hipGetLastError(), M);
+    const uint32_t zero = 0;
+    PTI_HIP(hipMemcpyAsync(nch.as<uint32_t>() + nseg, &zero, 4, hipMemcpyHostToDevice, s), M);
+    PTI_HIP(hipMemcpyAsync(nu.as<uint32_t>() + nseg, &zero, 4, hipMemcpyHostToDevice, s), M);
+    size_t sb3 = 0;
+    PTI_HIP(rocprim::exclusive_scan(nullptr, sb3, nch.as<uint32_t>(), choff.as<uint32_t>(), 0u, (size_t)nseg + 1,
+                                    rocprim::plus<uint32_t>(), s),
+            M);
+    PTI_HIP(tmp_seg.alloc(sb3), M);
+    PTI_HIP(rocprim::exclusive_scan(tmp_seg.ptr, sb3, nch.as<uint32_t>(), choff.as<uint32_t>(), 0u, (size_t)nseg + 1,
+                                    rocprim::plus<uint32_t>(), s),
+            M);
+    PTI_HIP(rocprim::exclusive_scan(tmp_seg.ptr, sb3, nu.as<uint32_t>(), uoff.as<uint32_t>(), 0u, (size_t)nseg + 1,
+                                    rocprim::plus<uint32_t>(), s),
+            M);
+    uint32_t tot[2] = {0, 0};
+    PTI_HIP(hipMemcpyAsync(&tot[0], choff.as<uint32_t>() + nseg, 4, hipMemcpyDeviceToHost, s), M);
+    PTI_HIP(hipMemcpyAsync(&tot[1], uoff.as<uint32_t>() + nseg, 4, hipMemcpyDeviceToHost, s), M);
+    PTI_HIP(hipStreamSynchronize(s), M);
+    c.nchunks = tot[0];
+    c.nunits = tot[1];
+    PTI_HIP(c.chunk_z.alloc((size_t)(c.nchunks + 1) * 4), M);
+    PTI_HIP(c.chunk_blk.alloc((size_t)c.nchunks * 4), M);
+    PTI_HIP(c.unit_chunk.alloc((size_t)(c.nunits + 1) * 4), M);
+    PTI_HIP(c.unit_br.alloc((size_t)c.nunits * 4), M);
+    PTI_HIP(c.unit_excl.alloc((size_t)c.nunits * 4), M);
+    hipLaunchKernelGGL(chunk_fill_kernel, grid_n(c.nchunks), dim3(256), 0, s, choff.as<uint32_t>(),
+                       seg_first.as<uint32_t>(), hb.as<uint32_t>(), nseg, c.nchunks, c.chunk_z.as<uint32_t>(),
+                       c.chunk_blk.as<uint32_t>());
+    PTI_HIP(hipGetLastError(), M);
+    PTI_HIP(hipMemcpyAsync(c.chunk_z.as<uint32_t>() + c.nchunks, &nnz32, 4, hipMemcpyHostToDevice, s), M);
+    hipLaunchKernelGGL(unit_fill_kernel, grid_n(c.nunits), dim3(256), 0, s, uoff.as<uint32_t>(), choff.as<uint32_t>(),
+                       seg_first.as<uint32_t>(), skey.as<uint32_t>(), nseg, c.nunits, c.unit_chunk.as<uint32_t>(),
+                       c.unit_br.as<uint32_t>(), c.unit_excl.as<uint32_t>());
+    PTI_HIP(hipGetLastError(), M);
+    PTI_HIP(hipMemcpyAsync(c.unit_chunk.as<uint32_t>() + c.nunits, &c.nchunks, 4, hipMemcpyHostToDevice, s), M);
+    // 4. Nonzeros, sorted by output row inside each chunk.
+    PTI_HIP(c.einds.alloc(nnz * EW * 4), M);
+    PTI_HIP(c.vals.alloc(nnz * 4), M);
+    PTI_HIP(c.lblk.alloc(nnz), M);
+    hipLaunchKernelGGL(chunk_sort_kernel, dim3((c.nchunks + 3) / 4), dim3(256), 0, s, r.bptr.as<uint64_t>(),
+                       sid.as<uint32_t>(), hb.as<uint32_t>(), nb, c.chunk_z.as<uint32_t>(), c.chunk_blk.as<uint32_t>(),
+                       c.nchunks, ep, r.vals.as<float>(), N, mode, t->sb_bits, c.einds.as<uint32_t>(),
+                       c.vals.as<float>(), c.lblk.as<uint8_t>());
+    PTI_HIP(hipGetLastError(), M);
+    // The host scalars copied above and every temporary buffer (released on
+    // return) must outlive the queued work.
+    PTI_HIP(hipStreamSynchronize(s), M);
+    t->bytes += nb * N * 4 + nnz * (EW * 4 + 5) + (size_t)(2 * c.nchunks + 1) * 4 + (size_t)(3 * c.nunits + 1) * 4;
+    c.ready = true;
+    return 0;
 }
 
 // Upload the copies of the modes in `mode_mask` (bit m = target mode m).
 int upload(ptiDevHiCOO **out, const ptiSparseTensorHiCOO *hitsr, uint32_t mode_mask, hipStream_t s) {
     static const char *M = "DevHiCOO Upload";
     PTI_CHECK(require_device(M), M);
-    PTI_CHECK(validate(hitsr, M), M);
+    RawHiCOO raw;
+    PTI_CHECK(raw_upload(raw, hitsr, s, M), M);
     ptiDevHiCOO *t = new ptiDevHiCOO();
     t->nmodes = hitsr->nmodes;
     t->sb_bits = hitsr->sb_bits;
@@ -245,7 +487,7 @@ int upload(ptiDevHiCOO **out, const ptiSparseTensorHiCOO *hitsr, uint32_t mode_m
     for (uint32_t m = 0; m < t->nmodes; ++m) t->ndims[m] = hitsr->ndims[m];
     for (uint32_t m = 0; m < t->nmodes; ++m) {
         if (!(mode_mask >> m & 1u)) continue;
-        int rc = build_mode(t, hitsr, m, s, M);
+        int rc = t->nnz ? build_mode(t, raw, m, s, M) : 0;
         if (rc) {
             delete t;
             return rc;
