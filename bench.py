@@ -100,40 +100,85 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(h, ndims, facs_np, R, sample_nnz, nthreads):
-    """The oracle's OpenMP HiCOO MTTKRP (superblock-parallel with per-element
-    atomics, src/sptensor/hicoo/mttkrp_omp.c:400-467) on the leading
-    superblocks holding about sample_nnz nonzeros; one warm-up and two timed
-    calls per mode.  Returns ms per full three-mode step, extrapolated per
-    nonzero."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libhiparti_ref.so")
+
+
+def _sample_prefix(h, sample_nnz):
+    """Smallest prefix of superblocks holding >= sample_nnz nonzeros:
+    (k_end superblocks, b_end blocks, z_end nonzeros)."""
     a = h.address
-    N = len(ndims)
     nk_total = int(a.kptr.len) - 1
     kptr = np.ctypeslib.as_array(a.kptr.data, shape=(nk_total + 1,))
     bptr_all = np.ctypeslib.as_array(a.bptr.data, shape=(int(a.bptr.len),))
-    # smallest prefix of superblocks with >= sample_nnz nonzeros
     k_end = int(np.searchsorted(bptr_all[kptr], sample_nnz))
     k_end = max(1, min(k_end, nk_total))
     b_end = int(kptr[k_end])
-    z_end = int(bptr_all[b_end])
-    sub = {
-        "kptr": kptr[:k_end + 1].copy(),
-        "bptr": bptr_all[:b_end + 1].copy(),
-        "binds": np.stack([np.ctypeslib.as_array(a.binds[m].data, shape=(b_end,)) for m in range(N)]),
-        "einds": np.stack([np.ctypeslib.as_array(a.einds[m].data, shape=(z_end,)) for m in range(N)]),
-        "values": np.ctypeslib.as_array(a.values.data, shape=(z_end,)).copy(),
-    }
+    return k_end, b_end, int(bptr_all[b_end])
+
+
+def _time_modes(run, N):
+    """One warm-up and two timed calls per mode; seconds per three-mode pass."""
     total = 0.0
     for mode in range(N):
-        O.mttkrp_hicoo(sub, int(a.sb_bits), facs_np, mode, kind="omp", nthreads=nthreads)
+        run(mode)
         t0 = time.perf_counter()
         for _ in range(2):
-            O.mttkrp_hicoo(sub, int(a.sb_bits), facs_np, mode, kind="omp", nthreads=nthreads)
+            run(mode)
         total += (time.perf_counter() - t0) / 2
-    per_nnz = total / z_end
-    return per_nnz * int(a.nnz) * 1e3, z_end, k_end
+    return total
+
+
+def cpu_baseline(h, ndims, facs_np, R, sample_nnz, nthreads):
+    """The CPU baseline on the leading superblocks holding about sample_nnz
+    nonzeros of the same HiCOO tensor, one warm-up and two timed calls per
+    mode, extrapolated per nonzero to the whole tensor (ms per three-mode
+    step).  It is the reference's own ptiOmpMTTKRPHiCOO with tk = host
+    threads and tb = 1 (superblock-parallel with per-element atomics,
+    src/sptensor/hicoo/mttkrp_omp.c:184-204,400-467) when its library has been
+    built from the reference sources (oracle/_ref), else the oracle's
+    restatement of that kernel.  Returns (ms, kind, z_end, k_end)."""
+    from hiparti_amd import _ffi as F
+    a = h.address
+    N = len(ndims)
+    k_end, b_end, z_end = _sample_prefix(h, sample_nnz)
+    if os.path.exists(REF_LIB):
+        L = C.CDLL(REF_LIB)
+        L.ptiOmpMTTKRPHiCOO.argtypes = [C.POINTER(F.ptiSparseTensorHiCOO), C.POINTER(C.POINTER(F.ptiMatrix)),
+                                        C.POINTER(F.ptiIndex), F.ptiIndex, C.c_int, C.c_int]
+        L.ptiOmpMTTKRPHiCOO.restype = C.c_int
+        view = F.ptiSparseTensorHiCOO.from_buffer_copy(a)   # same arrays, first k_end superblocks
+        view.kptr.len = k_end + 1
+        view.nnz = z_end
+        mats = []
+        for f in facs_np:
+            m = F.ptiMatrix(f.shape[0], f.shape[1], f.shape[0], f.shape[1], f.ctypes.data_as(C.POINTER(F.ptiValue)))
+            mats.append(m)
+        out = np.zeros((max(ndims), R), np.float32)
+        mats.append(F.ptiMatrix(max(ndims), R, max(ndims), R, out.ctypes.data_as(C.POINTER(F.ptiValue))))
+        ptrs = (C.POINTER(F.ptiMatrix) * (N + 1))(*[C.pointer(m) for m in mats])
+
+        def run(mode):
+            order = (F.ptiIndex * N)(*([mode] + [(mode + i) % N for i in range(1, N)]))
+            mats[N].nrows = ndims[mode]
+            assert L.ptiOmpMTTKRPHiCOO(C.byref(view), ptrs, order, mode, int(nthreads), 1) == 0
+        kind = "reference"
+    else:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        kptr = np.ctypeslib.as_array(a.kptr.data, shape=(int(a.kptr.len),))
+        sub = {
+            "kptr": kptr[:k_end + 1].copy(),
+            "bptr": np.ctypeslib.as_array(a.bptr.data, shape=(b_end + 1,)).copy(),
+            "binds": np.stack([np.ctypeslib.as_array(a.binds[m].data, shape=(b_end,)) for m in range(N)]),
+            "einds": np.stack([np.ctypeslib.as_array(a.einds[m].data, shape=(z_end,)) for m in range(N)]),
+            "values": np.ctypeslib.as_array(a.values.data, shape=(z_end,)).copy(),
+        }
+
+        def run(mode):
+            O.mttkrp_hicoo(sub, int(a.sb_bits), facs_np, mode, kind="omp", nthreads=nthreads)
+        kind = "port"
+    total = _time_modes(run, N)
+    return total / z_end * int(a.nnz) * 1e3, kind, z_end, k_end
 
 
 def cpd_leg(args):
@@ -294,11 +339,12 @@ def main():
         extra.update(cpd_leg(args))
     if not args.no_cpu:
         t0 = time.time()
-        cpu_ms, z_s, k_s = cpu_baseline(*cpu_args, args.cpu_sample_nnz, nthreads)
+        cpu_ms, kind, z_s, k_s = cpu_baseline(*cpu_args, args.cpu_sample_nnz, nthreads)
+        who = "reference ptiOmpMTTKRPHiCOO" if kind == "reference" else "oracle restatement of ptiOmpMTTKRPHiCOO"
         cpu = {"value": round(cpu_ms, 1), "unit": "ms per step (3-mode HiCOO MTTKRP, extrapolated per nonzero)",
-               "cores": nthreads, "kind": "port",
-               "sample": "oracle OpenMP HiCOO MTTKRP (tk=%d, tb=1) on the first %d superblocks = %d nonzeros of "
-                         "the same tensor, all 3 modes, 1 warm-up + 2 timed calls per mode" % (nthreads, k_s, z_s)}
+               "cores": nthreads, "kind": kind,
+               "sample": "%s (tk=%d, tb=1) on the first %d superblocks = %d nonzeros of the same tensor, all 3 "
+                         "modes, 1 warm-up + 2 timed calls per mode" % (who, nthreads, k_s, z_s)}
         log("cpu baseline %.1f s" % (time.time() - t0))
 
     line = {
