@@ -409,30 +409,42 @@ __device__ __forceinline__ void hicoo_slices(const FactorArgs &a, const HStage<N
 #pragma unroll
     for (int k = 0; k < V; ++k) acc[k] = 0.f;
     uint32_t cur = DUMMY_ROW;
-    for (uint32_t b = 0; b < per; b += D) {
+    // Consume slot j (position p), refill it with position p + D.
+    auto step = [&](int j, uint32_t next) {
+        float prod[V], t[V];
+        to_float(raw[j][0], prod);
 #pragma unroll
-        for (int j = 0; j < D; ++j) {
-            float prod[V], t[V];
-            to_float(raw[j][0], prod);
+        for (int k = 0; k < V; ++k) prod[k] *= vv[j];
 #pragma unroll
-            for (int k = 0; k < V; ++k) prod[k] *= vv[j];
+        for (int i = 1; i < N - 1; ++i) {
+            to_float(raw[j][i], t);
 #pragma unroll
-            for (int i = 1; i < N - 1; ++i) {
-                to_float(raw[j][i], t);
+            for (int k = 0; k < V; ++k) prod[k] *= t[k];
+        }
+        const uint32_t row = rr[j];
+        issue(j, next);
+        const bool brk = row != cur;
+        if (brk && cur != DUMMY_ROW) {
+            float *dst = tile + cur * tp + col;
 #pragma unroll
-                for (int k = 0; k < V; ++k) prod[k] *= t[k];
-            }
-            const uint32_t row = rr[j];
-            issue(j, (b + j + D) * NG + g);
-            const bool brk = row != cur;
-            if (brk && cur != DUMMY_ROW) {
-                float *dst = tile + cur * tp + col;
+            for (int k = 0; k < V; ++k) atomicAdd(dst + k, acc[k]);
+        }
 #pragma unroll
-                for (int k = 0; k < V; ++k) atomicAdd(dst + k, acc[k]);
-            }
+        for (int k = 0; k < V; ++k) acc[k] = brk ? prod[k] : acc[k] + prod[k];
+        cur = row;
+    };
+    constexpr int PMAX = CHUNK / NG;   // slice length of a full chunk
+    if constexpr (PMAX <= 32) {
+        // Straight-line walk over a full chunk's slice (shorter slices run
+        // into their dummy entries): with no loop back-edge the compiler's
+        // wait counts follow the ring exactly, keeping D slots of gathers in
+        // flight, and every LDS read has a constant offset.
 #pragma unroll
-            for (int k = 0; k < V; ++k) acc[k] = brk ? prod[k] : acc[k] + prod[k];
-            cur = row;
+        for (int p = 0; p < PMAX; ++p) step(p % D, (uint32_t)(p + D) * NG + g);
+    } else {
+        for (uint32_t b = 0; b < per; b += D) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) step(j, (b + j + D) * NG + g);
         }
     }
     if (cur != DUMMY_ROW) {
