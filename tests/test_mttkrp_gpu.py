@@ -212,3 +212,33 @@ def test_large_zipf_properties(oracle):
         f64 = oracle.mttkrp(inds, vals, facs, mode, kind="f64")
         assert rel_fro(h.MTTKRP(mode, dense_mats(facs, max(ndims))), f64) <= TOL
         assert rel_fro(t.MTTKRP(mode, dense_mats(facs, max(ndims))), f64) <= TOL
+
+
+def test_device_upload_rejects_malformed_hicoo():
+    """The device layout build validates the arrays it uses as memory and LDS
+    indices: element offsets < 2^sb_bits, coordinates inside their modes,
+    non-empty blocks."""
+    import ctypes as C
+    from hiparti_amd import PtiError
+    from hiparti_amd import _ffi as F
+    from hiparti_amd.device import DevHiCOO
+    rng = np.random.default_rng(17)
+    ndims = [60, 50, 40]
+    inds = np.stack([rng.integers(0, d, 3000) for d in ndims]).astype(np.uint32)
+    vals = rng.random(3000).astype(np.float32)
+    for corrupt in ("eind_too_big", "coord_outside", "empty_block"):
+        t = COOTensor.from_arrays(ndims, inds, vals)
+        h = t.toHiCOO(3, 4, 4)
+        a = h.address
+        nnz = int(a.nnz)
+        if corrupt == "eind_too_big":
+            np.ctypeslib.as_array(a.einds[1].data, shape=(nnz,))[nnz // 2] = 8      # sb_bits = 3
+        elif corrupt == "coord_outside":
+            nb = int(a.bptr.len) - 1
+            np.ctypeslib.as_array(a.binds[2].data, shape=(nb,))[nb // 3] = 5      # (5 << 3) >= 40
+        else:
+            bp = np.ctypeslib.as_array(a.bptr.data, shape=(int(a.bptr.len),))
+            bp[2] = bp[1]
+        with pytest.raises(PtiError) as e:
+            DevHiCOO(h)
+        assert e.value.code == F.PTIERR_VALUE_ERROR, corrupt
