@@ -142,7 +142,7 @@ int gpu_convert(ptiSparseTensorHiCOO *hitsr, ptiNnzIndex *max_nnzb, ptiSparseTen
     const uint32_t N = tsr->nmodes;
     const uint64_t nnz = tsr->nnz;
     if ((N != 3 && N != 4) || nnz < 2) return 1;
-    const std::vector<uint32_t> mx = pti::mode_maxima(tsr, 0);
+    const std::vector<uint32_t> mx = pti::mode_maxima(tsr, pti::default_threads());
     if (N == 3)
         for (uint32_t v : mx)
             if (v >= (1u << 24)) return 1;

@@ -18,9 +18,12 @@ inline int bit_width_u32(uint32_t v) {
 inline std::vector<uint32_t> mode_maxima(const ptiSparseTensor *tsr, int nthreads) {
     const ptiIndex N = tsr->nmodes;
     std::vector<uint32_t> mx(N, 0);
+    // parallel_for resolves nthreads <= 0 to default_threads(); size the
+    // per-thread slots for the count it will actually run.
+    if (nthreads <= 0) nthreads = default_threads();
     for (ptiIndex m = 0; m < N; ++m) {
         const uint32_t *d = tsr->inds[m].data;
-        std::vector<uint32_t> part(std::max(nthreads, 1), 0);
+        std::vector<uint32_t> part(nthreads, 0);
         parallel_for(tsr->nnz, nthreads, [&](uint64_t b, uint64_t e, int t) {
             uint32_t v = 0;
             for (uint64_t z = b; z < e; ++z) v = d[z] > v ? d[z] : v;
