@@ -171,12 +171,21 @@ __device__ __forceinline__ void wave_sync() {
 // block-row (the target mode's element offset) and the value in one 8-byte
 // word, and for each factor in product order (mats_order[1..N-1]) the row's
 // offset in units of the lane vector (coord * stride / V, checked on the
-// host to fit 32 bits).  Entries past a group's slice, up to per + 2 * D
-// positions, are dummies (invalid row, offset 0) so that the pipelined walk
-// can read and gather ahead without bounds tests.
+// host to fit 32 bits).  Entries past a group's slice, up to the last
+// position the walk reads (hicoo_straight), are dummies (invalid row, offset
+// 0) so that the pipelined walk can read and gather ahead without bounds
+// tests; every entry it reads is written for the current chunk.
 template <int NG, int D>
 constexpr int hstage_entries() {
     return CHUNK + NG + 2 * D * NG;
+}
+
+// The slice walk runs straight-line over a full chunk's slice (PMAX = CHUNK
+// / NG positions plus D of look-ahead, whatever the chunk's length) when
+// PMAX <= 32, else it loops over the slice's own length.
+template <int NG>
+constexpr bool hicoo_straight() {
+    return CHUNK / NG <= 32;
 }
 
 template <int N, int E>
@@ -193,6 +202,7 @@ template <int N, int V, int NG, int D, int E>
 __device__ __forceinline__ uint32_t stage_hicoo(const HiCOOArgs &h, const FactorArgs &a, uint32_t chunk,
                                                 HStage<N, E> &st, int lane) {
     constexpr int EW = (N + 3) / 4;
+    static_assert((CHUNK / NG + D) * NG <= E && (CHUNK / NG + 2 * D) * NG <= E, "staging too small for the walk");
     const uint32_t z0 = h.chunk_z[chunk];
     const uint32_t cnt = h.chunk_z[chunk + 1] - z0;
     const uint32_t b0 = h.chunk_blk[chunk];
@@ -216,8 +226,10 @@ __device__ __forceinline__ uint32_t stage_hicoo(const HiCOOArgs &h, const Factor
             st.off[i - 1][e] = ((h.binds[b * N + m] << h.sb_bits) + elem(m)) * vstride;
         }
     }
-    // Dummies: local positions at or past each group's slice length.
-    const uint32_t span = (per + 2 * D) * NG;
+    // Dummies: local positions at or past each group's slice length, up to
+    // the last position the walk reads (PMAX + D positions when it runs
+    // straight-line, per + 2 * D when it loops).
+    const uint32_t span = hicoo_straight<NG>() ? (uint32_t)(CHUNK / NG + D) * NG : (per + 2 * D) * NG;
     for (uint32_t e = lane; e < span; e += 64) {
         const uint32_t g = e % NG, j = e / NG;
         const uint32_t len = g * per < cnt ? (cnt - g * per < per ? cnt - g * per : per) : 0u;
@@ -434,7 +446,7 @@ __device__ __forceinline__ void hicoo_slices(const FactorArgs &a, const HStage<N
         cur = row;
     };
     constexpr int PMAX = CHUNK / NG;   // slice length of a full chunk
-    if constexpr (PMAX <= 32) {
+    if constexpr (hicoo_straight<NG>()) {
         // Straight-line walk over a full chunk's slice (shorter slices run
         // into their dummy entries): with no loop back-edge the compiler's
         // wait counts follow the ring exactly, keeping D slots of gathers in
