@@ -232,8 +232,9 @@ __device__ __forceinline__ uint32_t stage_hicoo(const HiCOOArgs &h, const Factor
     const uint32_t span = hicoo_straight<NG>() ? (uint32_t)(CHUNK / NG + D) * NG : (per + 2 * D) * NG;
     for (uint32_t e = lane; e < span; e += 64) {
         const uint32_t g = e % NG, j = e / NG;
-        const uint32_t len = g * per < cnt ? (cnt - g * per < per ? cnt - g * per : per) : 0u;
-        if (j >= len) {
+        // Position j of slice g holds nonzero g * per + j when j < per and
+        // that nonzero exists (no subtraction, so nothing can wrap).
+        if (j >= per || g * per + j >= cnt) {
             st.row[e] = DUMMY_ROW;
             st.val[e] = 0.f;
 #pragma unroll
