@@ -23,6 +23,11 @@
 // MTTKRP streams its copy sequentially, reduces equal rows in registers and
 // accumulates outputs in LDS.  The host HiCOO struct is not referenced after
 // the upload.
+//
+// Order-3 copies also get a stream form on first use (ensure_stream): the
+// nonzeros of each block-row sorted by output row and decoded into 16-byte
+// entries laid out for the lane groups of the rank shape (StreamArgs in
+// mttkrp_kernels.hpp), 4 KB per chunk, read by hicoo_stream_kernel.
 #include <atomic>
 #include <mutex>
 #include <unordered_map>
@@ -46,6 +51,11 @@ struct ptiDevHiCOOMode {
     pti::dev::Buffer binds, einds, vals, lblk, chunk_z, chunk_blk, unit_chunk, unit_br, unit_excl;
     uint32_t nchunks = 0, nunits = 0;
     bool ready = false;
+    // Stream form of an order-3 copy (16 bytes per chunk position), built on
+    // first use for the lane-group count `stream_ng` of the rank shape.
+    mutable std::mutex stream_mu;
+    mutable pti::dev::Buffer stream;
+    mutable uint32_t stream_ng = 0;
 };
 
 struct ptiDevHiCOO {
@@ -293,6 +303,53 @@ __global__ void __launch_bounds__(256) chunk_sort_kernel(const uint64_t *__restr
     }
 }
 
+// Stream-form build, step 1 (one wave per chunk): for each nonzero of the
+// mode copy its block and its output row (the target-mode coordinate).
+__global__ void __launch_bounds__(256) stream_keys_kernel(HiCOOArgs h, uint32_t nchunks, uint32_t mode,
+                                                          uint32_t *__restrict__ key, uint32_t *__restrict__ pos,
+                                                          uint32_t *__restrict__ blk_of) {
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t c = blockIdx.x * 4 + wid;
+    if (c >= nchunks) return;   // whole wave
+    const uint32_t z0 = h.chunk_z[c], cnt = h.chunk_z[c + 1] - z0, b0 = h.chunk_blk[c];
+    for (uint32_t zl = lane; zl < cnt; zl += 64) {
+        const uint32_t z = z0 + zl;
+        const uint32_t b = b0 + h.lblk[z];
+        blk_of[z] = b;
+        key[z] = (h.binds[(uint64_t)b * 3 + mode] << h.sb_bits) + ((h.einds[z] >> (mode * 8)) & 0xFFu);
+        pos[z] = z;
+    }
+}
+
+// Step 2, after a stable sort of the nonzeros by output row (each block-row
+// keeps its place, since block-rows are in ascending block coordinate, so the
+// chunk boundaries are unchanged): the entries of chunk c for NG lane groups
+// (one wave per chunk).  lo < hi are the two non-target modes.
+__global__ void __launch_bounds__(256) stream_fill_kernel(HiCOOArgs h, uint32_t nchunks, uint32_t mode, uint32_t lo,
+                                                          uint32_t hi, uint32_t ng, const uint32_t *__restrict__ spos,
+                                                          const uint32_t *__restrict__ blk_of,
+                                                          uint4 *__restrict__ ent) {
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t c = blockIdx.x * 4 + wid;
+    if (c >= nchunks) return;   // whole wave
+    const uint32_t z0 = h.chunk_z[c], cnt = h.chunk_z[c + 1] - z0;
+    const uint32_t per = (cnt + ng - 1) / ng;
+    for (uint32_t e = lane; e < CHUNK; e += 64) {
+        const uint32_t g = e % ng, j = e / ng;
+        uint4 x = make_uint4(0u, 0u, 0u, DUMMY_ROW);
+        if (j < per && g * per + j < cnt) {
+            const uint32_t z = spos[z0 + g * per + j];
+            const uint64_t b = blk_of[z];
+            const uint32_t w = h.einds[z];   // order 3: one packed word
+            x.x = (h.binds[b * 3 + lo] << h.sb_bits) + ((w >> (lo * 8)) & 0xFFu);
+            x.y = (h.binds[b * 3 + hi] << h.sb_bits) + ((w >> (hi * 8)) & 0xFFu);
+            x.z = __float_as_uint(h.vals[z]);
+            x.w = (w >> (mode * 8)) & 0xFFu;
+        }
+        ent[(uint64_t)c * CHUNK + e] = x;
+    }
+}
+
 template <typename T>
 int dev_put(Buffer &b, const T *src, size_t n, hipStream_t s, const char *M) {
     PTI_HIP(b.alloc(n * sizeof(T)), M);
@@ -499,8 +556,56 @@ int upload(ptiDevHiCOO **out, const ptiSparseTensorHiCOO *hitsr, uint32_t mode_m
 
 }  // namespace
 
+namespace {
+// Builds (or rebuilds, for another lane-group count) the stream form of the
+// order-3 copy of `mode`.
+int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t s, const char *M) {
+    const ptiDevHiCOOMode &c = t->mc[mode];
+    std::lock_guard<std::mutex> lk(c.stream_mu);
+    if (c.stream_ng == ng) return 0;
+    // Work queued on the old form must finish before it is released.
+    if (c.stream.ptr) PTI_HIP(hipDeviceSynchronize(), M);
+    PTI_HIP(c.stream.alloc((size_t)c.nchunks * CHUNK * sizeof(uint4)), M);
+    uint32_t lo = 3, hi = 0;
+    for (uint32_t m = 0; m < 3; ++m)
+        if (m != mode) {
+            lo = std::min(lo, m);
+            hi = std::max(hi, m);
+        }
+    const uint64_t nnz = t->nnz;
+    Buffer key, pos, skey, spos, blk_of, tmp;
+    PTI_HIP(key.alloc(nnz * 4), M);
+    PTI_HIP(pos.alloc(nnz * 4), M);
+    PTI_HIP(skey.alloc(nnz * 4), M);
+    PTI_HIP(spos.alloc(nnz * 4), M);
+    PTI_HIP(blk_of.alloc(nnz * 4), M);
+    const dim3 grid((c.nchunks + 3) / 4);
+    hipLaunchKernelGGL(stream_keys_kernel, grid, dim3(256), 0, s, hicoo_args(t, mode), c.nchunks, mode,
+                       key.as<uint32_t>(), pos.as<uint32_t>(), blk_of.as<uint32_t>());
+    PTI_HIP(hipGetLastError(), M);
+    unsigned kbits = 1;
+    while (kbits < 32 && ((uint64_t)t->ndims[mode] - 1) >> kbits) ++kbits;
+    size_t tb = 0;
+    PTI_HIP(rocprim::radix_sort_pairs(nullptr, tb, key.as<uint32_t>(), skey.as<uint32_t>(), pos.as<uint32_t>(),
+                                      spos.as<uint32_t>(), (size_t)nnz, 0, kbits, s),
+            M);
+    PTI_HIP(tmp.alloc(tb), M);
+    PTI_HIP(rocprim::radix_sort_pairs(tmp.ptr, tb, key.as<uint32_t>(), skey.as<uint32_t>(), pos.as<uint32_t>(),
+                                      spos.as<uint32_t>(), (size_t)nnz, 0, kbits, s),
+            M);
+    hipLaunchKernelGGL(stream_fill_kernel, grid, dim3(256), 0, s, hicoo_args(t, mode), c.nchunks, mode, lo, hi, ng,
+                       spos.as<uint32_t>(), blk_of.as<uint32_t>(), c.stream.as<uint4>());
+    PTI_HIP(hipGetLastError(), M);
+    // The temporaries are released on return: the queued work must finish.
+    PTI_HIP(hipStreamSynchronize(s), M);
+    c.stream_ng = ng;
+    return 0;
+}
+}  // namespace
+
 // Shared by the FP32 and BF16 entry points: zero the output rows, then run the
-// tile kernel once per 64-rank column tile.
+// stream kernel (order 3, factor matrices < 4 GiB) or the tile kernel once
+// per 64-rank column tile.
 int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiIndex R, ptiIndex stride,
               ptiIndex out_stride, const ptiIndex mats_order[], ptiIndex mode, bool bf16, hipStream_t s,
               const char *M) {
@@ -534,7 +639,25 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
         a.out_stride = out_stride;
         a.mode = mode;
         for (uint32_t i = 0; i < N; ++i) a.order[i] = mats_order[i];
-        PTI_HIP(launch_hicoo(N, hicoo_args(t, mode), a, sh, bf16, s), M);
+        bool fits = true;   // the stream kernel gathers with 32-bit byte offsets
+        for (uint32_t i = 1; i < N; ++i)
+            if ((uint64_t)t->ndims[mats_order[i]] * stride * esz >= (1ull << 32)) fits = false;
+        if (N == 3 && fits) {
+            PTI_CHECK(ensure_stream(t, mode, 64u / sh.G, s, M), M);
+            const ptiDevHiCOOMode &c = t->mc[mode];
+            StreamArgs h;
+            h.ent = c.stream.as<uint4>();
+            h.unit_chunk = c.unit_chunk.as<uint32_t>();
+            h.unit_br = c.unit_br.as<uint32_t>();
+            h.unit_excl = c.unit_excl.as<uint32_t>();
+            h.nunits = c.nunits;
+            h.sb_bits = t->sb_bits;
+            h.nrows = t->ndims[mode];
+            h.swap = mats_order[1] > mats_order[2] ? 1u : 0u;
+            PTI_HIP(launch_stream_3(h, a, sh, bf16, s), M);
+        } else {
+            PTI_HIP(launch_hicoo(N, hicoo_args(t, mode), a, sh, bf16, s), M);
+        }
     }
     return 0;
 }
@@ -567,7 +690,13 @@ int ptiDevHiCOOInfo(ptiDevHiCOO const *t, uint64_t *nnz, uint64_t *nblocks, uint
     if (nnz) *nnz = t->nnz;
     if (nblocks) *nblocks = t->nb;
     if (nmodes) *nmodes = t->nmodes;
-    if (bytes) *bytes = t->bytes;
+    if (bytes) {
+        *bytes = t->bytes;
+        for (uint32_t m = 0; m < t->nmodes; ++m) {
+            std::lock_guard<std::mutex> lk(t->mc[m].stream_mu);
+            *bytes += t->mc[m].stream.bytes;
+        }
+    }
     return 0;
 }
 

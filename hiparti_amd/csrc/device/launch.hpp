@@ -42,6 +42,9 @@ inline bool choose_shape(uint32_t R, uint32_t stride, uint32_t out_stride, Shape
     hipError_t launch_completion_##N(const COOArgs &c, const FactorArgs &a, Shape sh, float *resid,    \
                                      double *partials, hipStream_t s);
 
+// Order-3 HiCOO MTTKRP on the stream form (built into the order-3 object).
+hipError_t launch_stream_3(const StreamArgs &h, const FactorArgs &a, Shape sh, bool bf16, hipStream_t s);
+
 PTI_DECLARE_LAUNCHERS(2)
 PTI_DECLARE_LAUNCHERS(3)
 PTI_DECLARE_LAUNCHERS(4)

@@ -58,6 +58,34 @@ hipError_t hicoo_vg(const HiCOOArgs &h, const FactorArgs &a, bool bf16, hipStrea
 }
 
 template <int V, int G>
+hipError_t stream_vg(const StreamArgs &h, const FactorArgs &a, bool bf16, hipStream_t s) {
+    if constexpr (G * V > 64) {
+        return hipErrorInvalidValue;  // the LDS tile holds at most 64 ranks
+    } else {
+        if (h.nunits == 0) return hipSuccess;
+        const size_t lds = (size_t)(1u << h.sb_bits) * (G * V + 1) * sizeof(float);
+        dim3 grid(h.nunits);
+        if (bf16)
+            hipLaunchKernelGGL((hicoo_stream_kernel<V, G, uint16_t, 8>), grid, dim3(512), lds, s, h, a);
+        else
+            hipLaunchKernelGGL((hicoo_stream_kernel<V, G, float, 8>), grid, dim3(512), lds, s, h, a);
+        return hipGetLastError();
+    }
+}
+
+template <int V>
+hipError_t stream_v(const StreamArgs &h, const FactorArgs &a, int G, bool bf16, hipStream_t s) {
+    switch (G) {
+        case 4: return stream_vg<V, 4>(h, a, bf16, s);
+        case 8: return stream_vg<V, 8>(h, a, bf16, s);
+        case 16: return stream_vg<V, 16>(h, a, bf16, s);
+        case 32: return stream_vg<V, 32>(h, a, bf16, s);
+        case 64: return stream_vg<V, 64>(h, a, bf16, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int V, int G>
 hipError_t coo_vg(const COOArgs &c, const FactorArgs &a, hipStream_t s) {
     if (c.nchunks == 0) return hipSuccess;
     dim3 grid(grid_for_chunks(c.nchunks)), block(256);
@@ -120,6 +148,17 @@ hipError_t PTI_CAT(launch_hicoo_, PTI_NMODES)(const HiCOOArgs &h, const FactorAr
         default: return hipErrorInvalidValue;
     }
 }
+
+#if PTI_NMODES == 3
+hipError_t launch_stream_3(const StreamArgs &h, const FactorArgs &a, Shape sh, bool bf16, hipStream_t s) {
+    switch (sh.V) {
+        case 4: return stream_v<4>(h, a, sh.G, bf16, s);
+        case 2: return stream_v<2>(h, a, sh.G, bf16, s);
+        case 1: return stream_v<1>(h, a, sh.G, bf16, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+#endif
 
 hipError_t PTI_CAT(launch_coo_, PTI_NMODES)(const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s) {
     switch (sh.V) {

@@ -25,6 +25,10 @@
 // the number of nonzeros, and a Zipf head row no longer serialises millions
 // of atomics.
 //
+// Stream kernel (hicoo_stream_kernel), order 3: the same units and LDS tile,
+// but each chunk is read already decoded from HBM (see StreamArgs) and the
+// nonzeros of a block-row are in output-row order across its chunks.
+//
 // COO kernel (coo_mttkrp_kernel).  Per target mode the device keeps a copy
 // sorted by the output index; each run of equal rows in a chunk is added to
 // the output with one FP32 atomic per rank.
@@ -516,6 +520,156 @@ template <int N, int V, int G>
 constexpr size_t hicoo_stage_bytes(int W) {
     return (size_t)W * sizeof(HStage<N, hstage_entries<64 / G, PipeDepth<N, V>::value>()>);
 }
+
+// Stream form of an order-3 mode copy (hicoo_stream_kernel): decoded
+// nonzeros kept in HBM, so that an MTTKRP neither decodes block coordinates
+// nor stages chunks in LDS.  Each block-row's nonzeros are sorted (stably) by
+// output row over the whole block-row, not only inside a chunk, so a row's
+// nonzeros are consecutive and most of its contributions are summed in
+// registers; the chunk and unit boundaries are those of the copy.  Chunk c
+// owns entries [c * CHUNK, (c + 1) * CHUNK): its nonzeros are cut into NG
+// slices of per = ceil(cnt / NG) positions and position j of slice g is entry
+// j * NG + g, so the NG lane groups of a wave reading their j-th positions
+// load 16 * NG contiguous bytes.  An entry is {coordinate of the lower
+// non-target mode, coordinate of the higher one, value bits, output row
+// inside the block-row}; positions past a slice's end are {0, 0, 0,
+// DUMMY_ROW}.
+struct StreamArgs {
+    const uint4 *ent;
+    const uint32_t *unit_chunk;
+    const uint32_t *unit_br;
+    const uint32_t *unit_excl;
+    uint32_t nunits;
+    uint32_t sb_bits;
+    uint32_t nrows;
+    uint32_t swap;               // 1: the product's first factor is the higher non-target mode
+};
+
+// Walk one chunk of the stream form: each lane group runs through its slice
+// (PMAX positions, padded with dummies), gathers D positions ahead and loads
+// entries RS - D positions ahead of their gathers, all in straight-line code
+// so that every wait count is exact.  Runs of equal rows are summed in
+// registers and added to the LDS tile once per slice, as in hicoo_slices.
+template <int V, int G, int D, int RS, typename FT>
+__device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__restrict__ ent, uint32_t swap,
+                                            float *tile, uint32_t tp, int lane) {
+    using VT = typename Vec<FT, V>::T;
+    constexpr int NG = 64 / G;
+    constexpr int PMAX = CHUNK / NG;
+    static_assert(RS > D, "the entry ring must run ahead of the gathers");
+    const uint32_t g = (uint32_t)lane / G;
+    const uint32_t col = (uint32_t)(lane % G) * V;
+    // Uniform bases and 32-bit byte offsets (the host checks that every
+    // factor matrix spans < 4 GiB), so each load takes one offset register.
+    const char *f0 = static_cast<const char *>(a.fac[a.order[1]]);
+    const char *f1 = static_cast<const char *>(a.fac[a.order[2]]);
+    const uint32_t rowb = a.stride * (uint32_t)sizeof(FT);
+    const uint32_t colb = (col < a.R ? col : 0u) * (uint32_t)sizeof(FT);
+    const uint32_t eoff = g * (uint32_t)sizeof(uint4);
+    const char *eb = reinterpret_cast<const char *>(ent);
+    uint4 en[RS];
+    VT raw[D][2];
+    auto load_entry = [&](int p) {
+        return *reinterpret_cast<const uint4 *>(eb + (uint64_t)(eoff + (uint32_t)(p * NG * sizeof(uint4))));
+    };
+#pragma unroll
+    for (int p = 0; p < RS && p < PMAX; ++p) en[p] = load_entry(p);
+    auto issue = [&](int p) {
+        const uint4 x = en[p % RS];
+        const uint32_t i0 = swap ? x.y : x.x, i1 = swap ? x.x : x.y;
+        raw[p % D][0] = *reinterpret_cast<const VT *>(f0 + (uint64_t)(i0 * rowb + colb));
+        raw[p % D][1] = *reinterpret_cast<const VT *>(f1 + (uint64_t)(i1 * rowb + colb));
+    };
+#pragma unroll
+    for (int p = 0; p < D && p < PMAX; ++p) issue(p);
+    float acc[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) acc[k] = 0.f;
+    uint32_t cur = DUMMY_ROW;
+#pragma unroll
+    for (int p = 0; p < PMAX; ++p) {
+        const uint4 x = en[p % RS];
+        float prod[V], t[V];
+        to_float(raw[p % D][0], prod);
+        const float v = __uint_as_float(x.z);
+#pragma unroll
+        for (int k = 0; k < V; ++k) prod[k] *= v;
+        to_float(raw[p % D][1], t);
+#pragma unroll
+        for (int k = 0; k < V; ++k) prod[k] *= t[k];
+        const uint32_t row = x.w;
+        if (p + D < PMAX) issue(p + D);
+        if (p + RS < PMAX) en[p % RS] = load_entry(p + RS);
+        const bool brk = row != cur;
+        if (brk && cur != DUMMY_ROW) {
+            float *dst = tile + cur * tp + col;
+#pragma unroll
+            for (int k = 0; k < V; ++k) atomicAdd(dst + k, acc[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < V; ++k) acc[k] = brk ? prod[k] : acc[k] + prod[k];
+        cur = row;
+    }
+    if (cur != DUMMY_ROW) {
+        float *dst = tile + cur * tp + col;
+#pragma unroll
+        for (int k = 0; k < V; ++k) atomicAdd(dst + k, acc[k]);
+    }
+}
+
+// Pipeline of the stream walk: gather depth D, entry ring RS (entries run
+// RS - D positions ahead of their gathers) and the minimum resident waves
+// per SIMD the register budget is sized for.  Measured on the 100M-nonzero
+// order-3 Zipf tensor at R = 32: D = 4 / RS = 8 at four waves per SIMD (100
+// VGPRs, no spills) beat D = 2 at eight waves, D = 3 at six and D = 6 at
+// four.
+template <int V, typename FT>
+struct StreamConfig {
+    static constexpr int D = 4;
+    static constexpr int RS = 8;
+    static constexpr int OCC = sizeof(FT) * V >= 16 ? 4 : 6;
+};
+
+// Order-3 HiCOO MTTKRP on the stream form: one workgroup per unit, W waves
+// taking the unit's chunks in turn, an LDS tile of the block-row's output
+// rows written back once (stores when the unit owns the block-row, else
+// atomics).  The factor gathers address a uniform base with 32-bit byte
+// offsets: the host sends factor matrices of 4 GiB or more to the tile
+// kernel.
+template <int V, int G, typename FT, int W>
+__global__ void __launch_bounds__(W * 64, (StreamConfig<V, FT>::OCC)) hicoo_stream_kernel(StreamArgs h, FactorArgs a) {
+    constexpr int D = StreamConfig<V, FT>::D;
+    constexpr int RS = StreamConfig<V, FT>::RS;
+    extern __shared__ float tile[];  // (1 << sb_bits) rows x TP floats
+    constexpr int GV = G * V;
+    constexpr int TP = GV + 1;
+    constexpr int NT = W * 64;
+    const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+    const uint32_t u = remap_block(blockIdx.x, gridDim.x);   // gridDim.x == nunits
+    const uint32_t rows = 1u << h.sb_bits;
+    for (uint32_t i = tid; i < rows * TP; i += NT) tile[i] = 0.f;
+    __syncthreads();
+    const uint32_t c0 = h.unit_chunk[u], c1 = h.unit_chunk[u + 1];
+    const uint32_t w0 = __builtin_amdgcn_readfirstlane(wid);
+    for (uint32_t c = c0 + w0; c < c1; c += W)
+        stream_walk<V, G, D, RS, FT>(a, h.ent + (uint64_t)c * CHUNK, h.swap, tile, TP, lane);
+    __syncthreads();
+    const uint32_t base = h.unit_br[u] << h.sb_bits;
+    const bool excl = h.unit_excl[u] != 0;
+    for (uint32_t i = tid; i < rows * GV; i += NT) {
+        const uint32_t r = i / GV, c = i % GV;
+        const uint32_t orow = base + r;
+        if (c >= a.R || orow >= h.nrows) continue;
+        const float v = tile[r * TP + c];
+        if (v == 0.f) continue;
+        float *dst = a.out + (uint64_t)orow * a.out_stride + c;
+        if (excl)
+            *dst = v;
+        else
+            unsafeAtomicAdd(dst, v);
+    }
+}
+
 
 template <int N, int V, int G, typename FT>
 __global__ void __launch_bounds__(256) coo_mttkrp_kernel(COOArgs c, FactorArgs a) {

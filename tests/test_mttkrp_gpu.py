@@ -242,3 +242,34 @@ def test_device_upload_rejects_malformed_hicoo():
         with pytest.raises(PtiError) as e:
             DevHiCOO(h)
         assert e.value.code == F.PTIERR_VALUE_ERROR, corrupt
+
+
+def test_order3_factors_of_4gib_take_the_tile_kernel():
+    """Order-3 MTTKRPs whose gathered factor matrices span 4 GiB or more run
+    the tile kernel (the stream kernel gathers with 32-bit byte offsets); mode
+    0 here gathers only small factors and runs the stream kernel.  Both
+    against the FP64 definition on the nonzeros' factor rows."""
+    import torch
+    from hiparti_amd.device import DevHiCOO
+    ndims = [1 << 25, 300, 200]     # mode-0 factor: 2^25 x 32 FP32 = 4 GiB
+    R = 32
+    rng = np.random.default_rng(23)
+    nnz = 20000
+    inds = np.stack([rng.integers(0, d, nnz) for d in ndims]).astype(np.uint32)
+    t = COOTensor.from_arrays(ndims, inds, rng.random(nnz).astype(np.float32))
+    dh = DevHiCOO(t.toHiCOO(7, 10, 10))
+    inds, vals = t.inds().astype(np.int64), t.values().astype(np.float64)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    dfacs = [torch.rand((d, R), generator=gen, device="cuda") for d in ndims]
+    rows = [dfacs[m][torch.from_numpy(inds[m]).cuda()].double().cpu().numpy() for m in range(3)]
+    for mode in range(3):
+        a, b = [m for m in range(3) if m != mode]
+        want = np.zeros((ndims[mode], R))
+        np.add.at(want, inds[mode], vals[:, None] * rows[a] * rows[b])
+        out = dh.mttkrp(dfacs, mode)
+        torch.cuda.synchronize()
+        touched = np.unique(inds[mode])
+        got = out[torch.from_numpy(touched).cuda()].cpu().numpy()
+        assert rel_fro(got, want[touched]) <= TOL, mode
+        assert float(out.abs().sum().item()) == pytest.approx(float(np.abs(want).sum()), rel=1e-5)
+        del out
