@@ -304,9 +304,10 @@ __global__ void __launch_bounds__(256) chunk_sort_kernel(const uint64_t *__restr
 }
 
 // Stream-form build, step 1 (one wave per chunk): for each nonzero of the
-// mode copy its block and its output row (the target-mode coordinate).
-__global__ void __launch_bounds__(256) stream_keys_kernel(HiCOOArgs h, uint32_t nchunks, uint32_t mode,
-                                                          uint32_t *__restrict__ key, uint32_t *__restrict__ pos,
+// mode copy its block and its sort key: output row (the target-mode
+// coordinate) in the high word, the coordinate of mode `sec` in the low word.
+__global__ void __launch_bounds__(256) stream_keys_kernel(HiCOOArgs h, uint32_t nchunks, uint32_t mode, uint32_t sec,
+                                                          uint64_t *__restrict__ key, uint32_t *__restrict__ pos,
                                                           uint32_t *__restrict__ blk_of) {
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t c = blockIdx.x * 4 + wid;
@@ -316,15 +317,18 @@ __global__ void __launch_bounds__(256) stream_keys_kernel(HiCOOArgs h, uint32_t 
         const uint32_t z = z0 + zl;
         const uint32_t b = b0 + h.lblk[z];
         blk_of[z] = b;
-        key[z] = (h.binds[(uint64_t)b * 3 + mode] << h.sb_bits) + ((h.einds[z] >> (mode * 8)) & 0xFFu);
+        const uint32_t row = (h.binds[(uint64_t)b * 3 + mode] << h.sb_bits) + ((h.einds[z] >> (mode * 8)) & 0xFFu);
+        const uint32_t sc = (h.binds[(uint64_t)b * 3 + sec] << h.sb_bits) + ((h.einds[z] >> (sec * 8)) & 0xFFu);
+        key[z] = ((uint64_t)row << 32) | sc;
         pos[z] = z;
     }
 }
 
-// Step 2, after a stable sort of the nonzeros by output row (each block-row
-// keeps its place, since block-rows are in ascending block coordinate, so the
-// chunk boundaries are unchanged): the entries of chunk c for NG lane groups
-// (one wave per chunk).  lo < hi are the two non-target modes.
+// Step 2, after a stable sort of the nonzeros by (output row, coordinate of
+// the larger non-target mode): each block-row keeps its place, since
+// block-rows are in ascending block coordinate, so the chunk boundaries are
+// unchanged.  Writes the entries of chunk c for NG lane groups (one wave per
+// chunk).  lo < hi are the two non-target modes.
 __global__ void __launch_bounds__(256) stream_fill_kernel(HiCOOArgs h, uint32_t nchunks, uint32_t mode, uint32_t lo,
                                                           uint32_t hi, uint32_t ng, const uint32_t *__restrict__ spos,
                                                           const uint32_t *__restrict__ blk_of,
@@ -574,23 +578,27 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
         }
     const uint64_t nnz = t->nnz;
     Buffer key, pos, skey, spos, blk_of, tmp;
-    PTI_HIP(key.alloc(nnz * 4), M);
+    PTI_HIP(key.alloc(nnz * 8), M);
     PTI_HIP(pos.alloc(nnz * 4), M);
-    PTI_HIP(skey.alloc(nnz * 4), M);
+    PTI_HIP(skey.alloc(nnz * 8), M);
     PTI_HIP(spos.alloc(nnz * 4), M);
     PTI_HIP(blk_of.alloc(nnz * 4), M);
     const dim3 grid((c.nchunks + 3) / 4);
-    hipLaunchKernelGGL(stream_keys_kernel, grid, dim3(256), 0, s, hicoo_args(t, mode), c.nchunks, mode,
-                       key.as<uint32_t>(), pos.as<uint32_t>(), blk_of.as<uint32_t>());
+    // Inside a row, nonzeros follow the coordinate of the larger non-target
+    // mode, whose factor rows are the likelier cache misses.
+    const uint32_t sec = t->ndims[hi] >= t->ndims[lo] ? hi : lo;
+    hipLaunchKernelGGL(stream_keys_kernel, grid, dim3(256), 0, s, hicoo_args(t, mode), c.nchunks, mode, sec,
+                       key.as<uint64_t>(), pos.as<uint32_t>(), blk_of.as<uint32_t>());
     PTI_HIP(hipGetLastError(), M);
     unsigned kbits = 1;
     while (kbits < 32 && ((uint64_t)t->ndims[mode] - 1) >> kbits) ++kbits;
+    kbits += 32;
     size_t tb = 0;
-    PTI_HIP(rocprim::radix_sort_pairs(nullptr, tb, key.as<uint32_t>(), skey.as<uint32_t>(), pos.as<uint32_t>(),
+    PTI_HIP(rocprim::radix_sort_pairs(nullptr, tb, key.as<uint64_t>(), skey.as<uint64_t>(), pos.as<uint32_t>(),
                                       spos.as<uint32_t>(), (size_t)nnz, 0, kbits, s),
             M);
     PTI_HIP(tmp.alloc(tb), M);
-    PTI_HIP(rocprim::radix_sort_pairs(tmp.ptr, tb, key.as<uint32_t>(), skey.as<uint32_t>(), pos.as<uint32_t>(),
+    PTI_HIP(rocprim::radix_sort_pairs(tmp.ptr, tb, key.as<uint64_t>(), skey.as<uint64_t>(), pos.as<uint32_t>(),
                                       spos.as<uint32_t>(), (size_t)nnz, 0, kbits, s),
             M);
     hipLaunchKernelGGL(stream_fill_kernel, grid, dim3(256), 0, s, hicoo_args(t, mode), c.nchunks, mode, lo, hi, ng,

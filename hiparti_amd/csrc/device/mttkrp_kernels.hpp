@@ -526,7 +526,9 @@ constexpr size_t hicoo_stage_bytes(int W) {
 // nor stages chunks in LDS.  Each block-row's nonzeros are sorted (stably) by
 // output row over the whole block-row, not only inside a chunk, so a row's
 // nonzeros are consecutive and most of its contributions are summed in
-// registers; the chunk and unit boundaries are those of the copy.  Chunk c
+// registers, and inside a row by the coordinate of the larger non-target mode,
+// so repeated factor rows are gathered back to back; the chunk and unit
+// boundaries are those of the copy.  Chunk c
 // owns entries [c * CHUNK, (c + 1) * CHUNK): its nonzeros are cut into NG
 // slices of per = ceil(cnt / NG) positions and position j of slice g is entry
 // j * NG + g, so the NG lane groups of a wave reading their j-th positions
@@ -569,8 +571,13 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
     const char *eb = reinterpret_cast<const char *>(ent);
     uint4 en[RS];
     VT raw[D][2];
+    // Entries are read once: non-temporal loads keep them from displacing
+    // factor rows in the caches.
+    typedef unsigned v4u __attribute__((ext_vector_type(4)));
     auto load_entry = [&](int p) {
-        return *reinterpret_cast<const uint4 *>(eb + (uint64_t)(eoff + (uint32_t)(p * NG * sizeof(uint4))));
+        const v4u t = __builtin_nontemporal_load(
+            reinterpret_cast<const v4u *>(eb + (uint64_t)(eoff + (uint32_t)(p * NG * sizeof(uint4)))));
+        return make_uint4(t.x, t.y, t.z, t.w);
     };
 #pragma unroll
     for (int p = 0; p < RS && p < PMAX; ++p) en[p] = load_entry(p);
