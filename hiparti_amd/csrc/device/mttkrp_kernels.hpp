@@ -547,18 +547,25 @@ struct StreamArgs {
     uint32_t swap;               // 1: the product's first factor is the higher non-target mode
 };
 
-// Walk one chunk of the stream form: each lane group runs through its slice
-// (PMAX positions, padded with dummies), gathers D positions ahead and loads
-// entries RS - D positions ahead of their gathers, all in straight-line code
-// so that every wait count is exact.  Runs of equal rows are summed in
-// registers and added to the LDS tile once per slice, as in hicoo_slices.
-template <int V, int G, int D, int RS, typename FT>
+// Walk one chunk of the stream form.  The chunk's CHUNK entries are loaded
+// once, one 16-byte entry per lane per wave-wide block (CHUNK / 64 blocks,
+// read non-temporally: each is used once): lane L of block b holds position
+// b * EB + L / NG of lane group L % NG, with EB = 64 / NG positions per block.
+// A lane group takes position p's entry from lane (p % EB) * NG + g of its
+// block with ds_bpermute, so entries cost one vector-memory instruction per
+// EB positions instead of one per position (the vector-memory data path is
+// this kernel's bound).  Each lane group runs through its slice (PMAX
+// positions, padded with dummies) in straight-line code, gathering D
+// positions ahead, so every wait count is exact.  Runs of equal rows are
+// summed in registers and added to the LDS tile once per slice.
+template <int V, int G, int D, typename FT>
 __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__restrict__ ent, uint32_t swap,
                                             float *tile, uint32_t tp, int lane) {
     using VT = typename Vec<FT, V>::T;
     constexpr int NG = 64 / G;
     constexpr int PMAX = CHUNK / NG;
-    static_assert(RS > D, "the entry ring must run ahead of the gathers");
+    constexpr int EB = 64 / NG;
+    constexpr int NB = CHUNK / 64;
     const uint32_t g = (uint32_t)lane / G;
     const uint32_t col = (uint32_t)(lane % G) * V;
     // Uniform bases and 32-bit byte offsets (the host checks that every
@@ -567,23 +574,23 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
     const char *f1 = static_cast<const char *>(a.fac[a.order[2]]);
     const uint32_t rowb = a.stride * (uint32_t)sizeof(FT);
     const uint32_t colb = (col < a.R ? col : 0u) * (uint32_t)sizeof(FT);
-    const uint32_t eoff = g * (uint32_t)sizeof(uint4);
-    const char *eb = reinterpret_cast<const char *>(ent);
-    uint4 en[RS];
-    VT raw[D][2];
-    // Entries are read once: non-temporal loads keep them from displacing
-    // factor rows in the caches.
     typedef unsigned v4u __attribute__((ext_vector_type(4)));
-    auto load_entry = [&](int p) {
-        const v4u t = __builtin_nontemporal_load(
-            reinterpret_cast<const v4u *>(eb + (uint64_t)(eoff + (uint32_t)(p * NG * sizeof(uint4)))));
-        return make_uint4(t.x, t.y, t.z, t.w);
-    };
+    const v4u *eb = reinterpret_cast<const v4u *>(ent) + lane;
+    v4u blk[NB];
 #pragma unroll
-    for (int p = 0; p < RS && p < PMAX; ++p) en[p] = load_entry(p);
+    for (int b = 0; b < NB; ++b) blk[b] = __builtin_nontemporal_load(eb + b * 64);
+    const int gsrc = (int)g * 4;
+    auto part = [&](int p, int w) -> uint32_t {
+        return (uint32_t)__builtin_amdgcn_ds_bpermute(gsrc + (p % EB) * NG * 4, (int)blk[p / EB][w]);
+    };
+    VT raw[D][2];
+    float vv[D];
+    uint32_t rr[D];
     auto issue = [&](int p) {
-        const uint4 x = en[p % RS];
-        const uint32_t i0 = swap ? x.y : x.x, i1 = swap ? x.x : x.y;
+        const uint32_t c0 = part(p, 0), c1 = part(p, 1);
+        vv[p % D] = __uint_as_float(part(p, 2));
+        rr[p % D] = part(p, 3);
+        const uint32_t i0 = swap ? c1 : c0, i1 = swap ? c0 : c1;
         raw[p % D][0] = *reinterpret_cast<const VT *>(f0 + (uint64_t)(i0 * rowb + colb));
         raw[p % D][1] = *reinterpret_cast<const VT *>(f1 + (uint64_t)(i1 * rowb + colb));
     };
@@ -595,18 +602,16 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
     uint32_t cur = DUMMY_ROW;
 #pragma unroll
     for (int p = 0; p < PMAX; ++p) {
-        const uint4 x = en[p % RS];
         float prod[V], t[V];
         to_float(raw[p % D][0], prod);
-        const float v = __uint_as_float(x.z);
+        const float v = vv[p % D];
 #pragma unroll
         for (int k = 0; k < V; ++k) prod[k] *= v;
         to_float(raw[p % D][1], t);
 #pragma unroll
         for (int k = 0; k < V; ++k) prod[k] *= t[k];
-        const uint32_t row = x.w;
+        const uint32_t row = rr[p % D];
         if (p + D < PMAX) issue(p + D);
-        if (p + RS < PMAX) en[p % RS] = load_entry(p + RS);
         const bool brk = row != cur;
         if (brk && cur != DUMMY_ROW) {
             float *dst = tile + cur * tp + col;
@@ -624,17 +629,15 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
     }
 }
 
-// Pipeline of the stream walk: gather depth D, entry ring RS (entries run
-// RS - D positions ahead of their gathers) and the minimum resident waves
+// Pipeline of the stream walk: gather depth D and the minimum resident waves
 // per SIMD the register budget is sized for.  Measured on the 100M-nonzero
-// order-3 Zipf tensor at R = 32: D = 4 / RS = 8 at four waves per SIMD (100
-// VGPRs, no spills) beat D = 2 at eight waves, D = 3 at six and D = 6 at
-// four.
+// order-3 Zipf tensor at R = 32 (FP32): D = 4 at five waves per SIMD (111
+// VGPRs) was the fastest of D = 2..8 at two to eight waves; deeper rings at
+// fewer waves were up to 10% slower.
 template <int V, typename FT>
 struct StreamConfig {
     static constexpr int D = 4;
-    static constexpr int RS = 8;
-    static constexpr int OCC = sizeof(FT) * V >= 16 ? 4 : 6;
+    static constexpr int OCC = sizeof(FT) * V >= 16 ? 5 : 6;
 };
 
 // Order-3 HiCOO MTTKRP on the stream form: one workgroup per unit, W waves
@@ -646,7 +649,6 @@ struct StreamConfig {
 template <int V, int G, typename FT, int W>
 __global__ void __launch_bounds__(W * 64, (StreamConfig<V, FT>::OCC)) hicoo_stream_kernel(StreamArgs h, FactorArgs a) {
     constexpr int D = StreamConfig<V, FT>::D;
-    constexpr int RS = StreamConfig<V, FT>::RS;
     extern __shared__ float tile[];  // (1 << sb_bits) rows x TP floats
     constexpr int GV = G * V;
     constexpr int TP = GV + 1;
@@ -659,7 +661,7 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<V, FT>::OCC)) hicoo_stre
     const uint32_t c0 = h.unit_chunk[u], c1 = h.unit_chunk[u + 1];
     const uint32_t w0 = __builtin_amdgcn_readfirstlane(wid);
     for (uint32_t c = c0 + w0; c < c1; c += W)
-        stream_walk<V, G, D, RS, FT>(a, h.ent + (uint64_t)c * CHUNK, h.swap, tile, TP, lane);
+        stream_walk<V, G, D, FT>(a, h.ent + (uint64_t)c * CHUNK, h.swap, tile, TP, lane);
     __syncthreads();
     const uint32_t base = h.unit_br[u] << h.sb_bits;
     const bool excl = h.unit_excl[u] != 0;
