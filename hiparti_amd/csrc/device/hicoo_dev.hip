@@ -24,10 +24,11 @@
 // accumulates outputs in LDS.  The host HiCOO struct is not referenced after
 // the upload.
 //
-// Order-3 copies also get a stream form on first use (ensure_stream): the
-// nonzeros of each block-row sorted by output row and decoded into 16-byte
-// entries laid out for the lane groups of the rank shape (StreamArgs in
-// mttkrp_kernels.hpp), 4 KB per chunk, read by hicoo_stream_kernel.
+// Each mode copy also gets a stream form on its first MTTKRP (ensure_stream):
+// the nonzeros of each block-row sorted by output row and decoded into
+// (N + 1)-word entries laid out for the lane groups of the rank shape
+// (StreamShape in mttkrp_kernels.hpp), 4 KB per chunk and 16-byte plane, read
+// by hicoo_stream_kernel.
 #include <atomic>
 #include <mutex>
 #include <unordered_map>
@@ -51,8 +52,8 @@ struct ptiDevHiCOOMode {
     pti::dev::Buffer binds, einds, vals, lblk, chunk_z, chunk_blk, unit_chunk, unit_br, unit_excl;
     uint32_t nchunks = 0, nunits = 0;
     bool ready = false;
-    // Stream form of an order-3 copy (16 bytes per chunk position), built on
-    // first use for the lane-group count `stream_ng` of the rank shape.
+    // Stream form of the copy (16 bytes per chunk position and plane), built
+    // on first use for the lane-group count `stream_ng` of the rank shape.
     mutable std::mutex stream_mu;
     mutable pti::dev::Buffer stream;
     mutable uint32_t stream_ng = 0;
@@ -306,51 +307,60 @@ __global__ void __launch_bounds__(256) chunk_sort_kernel(const uint64_t *__restr
 // Stream-form build, step 1 (one wave per chunk): for each nonzero of the
 // mode copy its block and its sort key: output row (the target-mode
 // coordinate) in the high word, the coordinate of mode `sec` in the low word.
-__global__ void __launch_bounds__(256) stream_keys_kernel(HiCOOArgs h, uint32_t nchunks, uint32_t mode, uint32_t sec,
-                                                          uint64_t *__restrict__ key, uint32_t *__restrict__ pos,
-                                                          uint32_t *__restrict__ blk_of) {
+__global__ void __launch_bounds__(256) stream_keys_kernel(HiCOOArgs h, uint32_t nchunks, uint32_t N, uint32_t mode,
+                                                          uint32_t sec, uint64_t *__restrict__ key,
+                                                          uint32_t *__restrict__ pos, uint32_t *__restrict__ blk_of) {
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t c = blockIdx.x * 4 + wid;
     if (c >= nchunks) return;   // whole wave
+    const uint32_t EW = (N + 3) / 4;
     const uint32_t z0 = h.chunk_z[c], cnt = h.chunk_z[c + 1] - z0, b0 = h.chunk_blk[c];
     for (uint32_t zl = lane; zl < cnt; zl += 64) {
         const uint32_t z = z0 + zl;
         const uint32_t b = b0 + h.lblk[z];
         blk_of[z] = b;
-        const uint32_t row = (h.binds[(uint64_t)b * 3 + mode] << h.sb_bits) + ((h.einds[z] >> (mode * 8)) & 0xFFu);
-        const uint32_t sc = (h.binds[(uint64_t)b * 3 + sec] << h.sb_bits) + ((h.einds[z] >> (sec * 8)) & 0xFFu);
-        key[z] = ((uint64_t)row << 32) | sc;
+        auto coord = [&](uint32_t m) {
+            return (h.binds[(uint64_t)b * N + m] << h.sb_bits) +
+                   ((h.einds[(uint64_t)z * EW + m / 4] >> ((m % 4) * 8)) & 0xFFu);
+        };
+        key[z] = ((uint64_t)coord(mode) << 32) | coord(sec);
         pos[z] = z;
     }
 }
 
 // Step 2, after a stable sort of the nonzeros by (output row, coordinate of
-// the larger non-target mode): each block-row keeps its place, since
+// the largest non-target mode): each block-row keeps its place, since
 // block-rows are in ascending block coordinate, so the chunk boundaries are
-// unchanged.  Writes the entries of chunk c for NG lane groups (one wave per
-// chunk).  lo < hi are the two non-target modes.
-__global__ void __launch_bounds__(256) stream_fill_kernel(HiCOOArgs h, uint32_t nchunks, uint32_t mode, uint32_t lo,
-                                                          uint32_t hi, uint32_t ng, const uint32_t *__restrict__ spos,
+// unchanged.  Writes the entries of chunk c for NG lane groups in the layout
+// of StreamShape (one wave per chunk).
+__global__ void __launch_bounds__(256) stream_fill_kernel(HiCOOArgs h, uint32_t nchunks, uint32_t N, uint32_t mode,
+                                                          uint32_t ng, const uint32_t *__restrict__ spos,
                                                           const uint32_t *__restrict__ blk_of,
                                                           uint4 *__restrict__ ent) {
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t c = blockIdx.x * 4 + wid;
     if (c >= nchunks) return;   // whole wave
+    const uint32_t EW = (N + 3) / 4, NP = (N + 4) / 4;
     const uint32_t z0 = h.chunk_z[c], cnt = h.chunk_z[c + 1] - z0;
     const uint32_t per = (cnt + ng - 1) / ng;
     for (uint32_t e = lane; e < CHUNK; e += 64) {
         const uint32_t g = e % ng, j = e / ng;
-        uint4 x = make_uint4(0u, 0u, 0u, DUMMY_ROW);
+        uint32_t w[12];
+        for (uint32_t k = 0; k < 12; ++k) w[k] = 0u;
+        w[N] = DUMMY_ROW;
         if (j < per && g * per + j < cnt) {
             const uint32_t z = spos[z0 + g * per + j];
             const uint64_t b = blk_of[z];
-            const uint32_t w = h.einds[z];   // order 3: one packed word
-            x.x = (h.binds[b * 3 + lo] << h.sb_bits) + ((w >> (lo * 8)) & 0xFFu);
-            x.y = (h.binds[b * 3 + hi] << h.sb_bits) + ((w >> (hi * 8)) & 0xFFu);
-            x.z = __float_as_uint(h.vals[z]);
-            x.w = (w >> (mode * 8)) & 0xFFu;
+            auto elem = [&](uint32_t m) { return (h.einds[(uint64_t)z * EW + m / 4] >> ((m % 4) * 8)) & 0xFFu; };
+            for (uint32_t i = 0; i + 1 < N; ++i) {
+                const uint32_t m = i < mode ? i : i + 1;
+                w[i] = (h.binds[b * N + m] << h.sb_bits) + elem(m);
+            }
+            w[N - 1] = __float_as_uint(h.vals[z]);
+            w[N] = elem(mode);
         }
-        ent[(uint64_t)c * CHUNK + e] = x;
+        for (uint32_t q = 0; q < NP; ++q)
+            ent[((uint64_t)c * NP + q) * CHUNK + e] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
     }
 }
 
@@ -569,13 +579,14 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
     if (c.stream_ng == ng) return 0;
     // Work queued on the old form must finish before it is released.
     if (c.stream.ptr) PTI_HIP(hipDeviceSynchronize(), M);
-    PTI_HIP(c.stream.alloc((size_t)c.nchunks * CHUNK * sizeof(uint4)), M);
-    uint32_t lo = 3, hi = 0;
-    for (uint32_t m = 0; m < 3; ++m)
-        if (m != mode) {
-            lo = std::min(lo, m);
-            hi = std::max(hi, m);
-        }
+    const uint32_t N = t->nmodes;
+    const uint32_t NP = (N + 4) / 4;
+    PTI_HIP(c.stream.alloc((size_t)c.nchunks * NP * CHUNK * sizeof(uint4)), M);
+    // Inside a row, nonzeros follow the coordinate of the largest non-target
+    // mode, whose factor rows are the likeliest cache misses.
+    uint32_t sec = mode == 0 ? 1 : 0;
+    for (uint32_t m = 0; m < N; ++m)
+        if (m != mode && t->ndims[m] >= t->ndims[sec]) sec = m;
     const uint64_t nnz = t->nnz;
     Buffer key, pos, skey, spos, blk_of, tmp;
     PTI_HIP(key.alloc(nnz * 8), M);
@@ -584,10 +595,7 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
     PTI_HIP(spos.alloc(nnz * 4), M);
     PTI_HIP(blk_of.alloc(nnz * 4), M);
     const dim3 grid((c.nchunks + 3) / 4);
-    // Inside a row, nonzeros follow the coordinate of the larger non-target
-    // mode, whose factor rows are the likelier cache misses.
-    const uint32_t sec = t->ndims[hi] >= t->ndims[lo] ? hi : lo;
-    hipLaunchKernelGGL(stream_keys_kernel, grid, dim3(256), 0, s, hicoo_args(t, mode), c.nchunks, mode, sec,
+    hipLaunchKernelGGL(stream_keys_kernel, grid, dim3(256), 0, s, hicoo_args(t, mode), c.nchunks, N, mode, sec,
                        key.as<uint64_t>(), pos.as<uint32_t>(), blk_of.as<uint32_t>());
     PTI_HIP(hipGetLastError(), M);
     unsigned kbits = 1;
@@ -601,7 +609,7 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
     PTI_HIP(rocprim::radix_sort_pairs(tmp.ptr, tb, key.as<uint64_t>(), skey.as<uint64_t>(), pos.as<uint32_t>(),
                                       spos.as<uint32_t>(), (size_t)nnz, 0, kbits, s),
             M);
-    hipLaunchKernelGGL(stream_fill_kernel, grid, dim3(256), 0, s, hicoo_args(t, mode), c.nchunks, mode, lo, hi, ng,
+    hipLaunchKernelGGL(stream_fill_kernel, grid, dim3(256), 0, s, hicoo_args(t, mode), c.nchunks, N, mode, ng,
                        spos.as<uint32_t>(), blk_of.as<uint32_t>(), c.stream.as<uint4>());
     PTI_HIP(hipGetLastError(), M);
     // The temporaries are released on return: the queued work must finish.
@@ -612,8 +620,8 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
 }  // namespace
 
 // Shared by the FP32 and BF16 entry points: zero the output rows, then run the
-// stream kernel (order 3, factor matrices < 4 GiB) or the tile kernel once
-// per 64-rank column tile.
+// stream kernel (factor matrices < 4 GiB) or the tile kernel once per rank
+// column tile.
 int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiIndex R, ptiIndex stride,
               ptiIndex out_stride, const ptiIndex mats_order[], ptiIndex mode, bool bf16, hipStream_t s,
               const char *M) {
@@ -650,7 +658,7 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
         bool fits = true;   // the stream kernel gathers with 32-bit byte offsets
         for (uint32_t i = 1; i < N; ++i)
             if ((uint64_t)t->ndims[mats_order[i]] * stride * esz >= (1ull << 32)) fits = false;
-        if (N == 3 && fits) {
+        if (fits && stream_supported(N, sh.G)) {
             PTI_CHECK(ensure_stream(t, mode, 64u / sh.G, s, M), M);
             const ptiDevHiCOOMode &c = t->mc[mode];
             StreamArgs h;
@@ -661,8 +669,7 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
             h.nunits = c.nunits;
             h.sb_bits = t->sb_bits;
             h.nrows = t->ndims[mode];
-            h.swap = mats_order[1] > mats_order[2] ? 1u : 0u;
-            PTI_HIP(launch_stream_3(h, a, sh, bf16, s), M);
+            PTI_HIP(launch_stream(N, h, a, sh, bf16, s), M);
         } else {
             PTI_HIP(launch_hicoo(N, hicoo_args(t, mode), a, sh, bf16, s), M);
         }

@@ -36,14 +36,17 @@ inline bool choose_shape(uint32_t R, uint32_t stride, uint32_t out_stride, Shape
     return true;
 }
 
+// Shapes the stream kernel runs: orders up to 5 (beyond, the entry planes and
+// the gathered rows no longer fit the register budget) and lane groups of at
+// most 16 lanes (a slice of at most 64 positions, walked straight-line).
+constexpr bool stream_supported(uint32_t N, int G) { return N <= 5 && G <= 16; }
+
 #define PTI_DECLARE_LAUNCHERS(N)                                                                       \
     hipError_t launch_hicoo_##N(const HiCOOArgs &h, const FactorArgs &a, Shape sh, bool bf16, hipStream_t s); \
+    hipError_t launch_stream_##N(const StreamArgs &h, const FactorArgs &a, Shape sh, bool bf16, hipStream_t s); \
     hipError_t launch_coo_##N(const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s);        \
     hipError_t launch_completion_##N(const COOArgs &c, const FactorArgs &a, Shape sh, float *resid,    \
                                      double *partials, hipStream_t s);
-
-// Order-3 HiCOO MTTKRP on the stream form (built into the order-3 object).
-hipError_t launch_stream_3(const StreamArgs &h, const FactorArgs &a, Shape sh, bool bf16, hipStream_t s);
 
 PTI_DECLARE_LAUNCHERS(2)
 PTI_DECLARE_LAUNCHERS(3)
@@ -69,6 +72,12 @@ PTI_DECLARE_LAUNCHERS(8)
 inline hipError_t launch_hicoo(uint32_t N, const HiCOOArgs &h, const FactorArgs &a, Shape sh, bool bf16,
                                hipStream_t s) {
 #define PTI_CALL(n) launch_hicoo_##n(h, a, sh, bf16, s)
+    PTI_DISPATCH_N(N, PTI_CALL)
+#undef PTI_CALL
+}
+inline hipError_t launch_stream(uint32_t N, const StreamArgs &h, const FactorArgs &a, Shape sh, bool bf16,
+                                hipStream_t s) {
+#define PTI_CALL(n) launch_stream_##n(h, a, sh, bf16, s)
     PTI_DISPATCH_N(N, PTI_CALL)
 #undef PTI_CALL
 }

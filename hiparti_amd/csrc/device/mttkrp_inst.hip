@@ -59,16 +59,16 @@ hipError_t hicoo_vg(const HiCOOArgs &h, const FactorArgs &a, bool bf16, hipStrea
 
 template <int V, int G>
 hipError_t stream_vg(const StreamArgs &h, const FactorArgs &a, bool bf16, hipStream_t s) {
-    if constexpr (G * V > 64) {
+    if constexpr (G * V > 64 || !stream_supported(N, G)) {
         return hipErrorInvalidValue;  // the LDS tile holds at most 64 ranks
     } else {
         if (h.nunits == 0) return hipSuccess;
         const size_t lds = (size_t)(1u << h.sb_bits) * (G * V + 1) * sizeof(float);
         dim3 grid(h.nunits);
         if (bf16)
-            hipLaunchKernelGGL((hicoo_stream_kernel<V, G, uint16_t, 8>), grid, dim3(512), lds, s, h, a);
+            hipLaunchKernelGGL((hicoo_stream_kernel<N, V, G, uint16_t, 8>), grid, dim3(512), lds, s, h, a);
         else
-            hipLaunchKernelGGL((hicoo_stream_kernel<V, G, float, 8>), grid, dim3(512), lds, s, h, a);
+            hipLaunchKernelGGL((hicoo_stream_kernel<N, V, G, float, 8>), grid, dim3(512), lds, s, h, a);
         return hipGetLastError();
     }
 }
@@ -149,8 +149,8 @@ hipError_t PTI_CAT(launch_hicoo_, PTI_NMODES)(const HiCOOArgs &h, const FactorAr
     }
 }
 
-#if PTI_NMODES == 3
-hipError_t launch_stream_3(const StreamArgs &h, const FactorArgs &a, Shape sh, bool bf16, hipStream_t s) {
+hipError_t PTI_CAT(launch_stream_, PTI_NMODES)(const StreamArgs &h, const FactorArgs &a, Shape sh, bool bf16,
+                                               hipStream_t s) {
     switch (sh.V) {
         case 4: return stream_v<4>(h, a, sh.G, bf16, s);
         case 2: return stream_v<2>(h, a, sh.G, bf16, s);
@@ -158,7 +158,6 @@ hipError_t launch_stream_3(const StreamArgs &h, const FactorArgs &a, Shape sh, b
         default: return hipErrorInvalidValue;
     }
 }
-#endif
 
 hipError_t PTI_CAT(launch_coo_, PTI_NMODES)(const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s) {
     switch (sh.V) {

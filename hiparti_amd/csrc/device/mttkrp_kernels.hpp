@@ -25,9 +25,11 @@
 // the number of nonzeros, and a Zipf head row no longer serialises millions
 // of atomics.
 //
-// Stream kernel (hicoo_stream_kernel), order 3: the same units and LDS tile,
-// but each chunk is read already decoded from HBM (see StreamArgs) and the
-// nonzeros of a block-row are in output-row order across its chunks.
+// Stream kernel (hicoo_stream_kernel): the same units and LDS tile, but each
+// chunk is read already decoded from HBM (see StreamShape) and the nonzeros
+// of a block-row are in output-row order across its chunks.  It runs orders
+// 2-5 with lane groups of at most 16 lanes and factor matrices below 4 GiB;
+// the tile kernel takes the rest.
 //
 // COO kernel (coo_mttkrp_kernel).  Per target mode the device keeps a copy
 // sorted by the output index; each run of equal rows in a chunk is added to
@@ -521,21 +523,28 @@ constexpr size_t hicoo_stage_bytes(int W) {
     return (size_t)W * sizeof(HStage<N, hstage_entries<64 / G, PipeDepth<N, V>::value>()>);
 }
 
-// Stream form of an order-3 mode copy (hicoo_stream_kernel): decoded
-// nonzeros kept in HBM, so that an MTTKRP neither decodes block coordinates
-// nor stages chunks in LDS.  Each block-row's nonzeros are sorted (stably) by
-// output row over the whole block-row, not only inside a chunk, so a row's
-// nonzeros are consecutive and most of its contributions are summed in
-// registers, and inside a row by the coordinate of the larger non-target mode,
-// so repeated factor rows are gathered back to back; the chunk and unit
-// boundaries are those of the copy.  Chunk c
-// owns entries [c * CHUNK, (c + 1) * CHUNK): its nonzeros are cut into NG
-// slices of per = ceil(cnt / NG) positions and position j of slice g is entry
-// j * NG + g, so the NG lane groups of a wave reading their j-th positions
-// load 16 * NG contiguous bytes.  An entry is {coordinate of the lower
-// non-target mode, coordinate of the higher one, value bits, output row
-// inside the block-row}; positions past a slice's end are {0, 0, 0,
-// DUMMY_ROW}.
+// Stream form of a mode copy (hicoo_stream_kernel): decoded nonzeros kept in
+// HBM, so that an MTTKRP neither decodes block coordinates nor stages chunks
+// in LDS.  Each block-row's nonzeros are sorted (stably) by output row over
+// the whole block-row, not only inside a chunk, so a row's nonzeros are
+// consecutive and most of its contributions are summed in registers, and
+// inside a row by the coordinate of the largest non-target mode, so repeated
+// factor rows are gathered back to back; the chunk and unit boundaries are
+// those of the copy.  An entry has N + 1 words: the coordinates of the N - 1
+// non-target modes in ascending mode order, the value bits and the output row
+// inside the block-row; positions past a slice's end are all-zero with row
+// DUMMY_ROW.  Entries are stored in NP = ceil((N + 1) / 4) planes of 16-byte
+// words.  Chunk c owns NP * CHUNK 16-byte words starting at c * NP * CHUNK,
+// plane q first at offset q * CHUNK: its nonzeros are cut into NG slices of
+// per = ceil(cnt / NG) positions and position j of slice g is entry j * NG +
+// g, so 64 consecutive entries of one plane are one wave-wide 16-byte-per-lane
+// load.
+template <int N>
+struct StreamShape {
+    static constexpr int WORDS = N + 1;
+    static constexpr int NP = (WORDS + 3) / 4;
+};
+
 struct StreamArgs {
     const uint4 *ent;
     const uint32_t *unit_chunk;
@@ -544,24 +553,26 @@ struct StreamArgs {
     uint32_t nunits;
     uint32_t sb_bits;
     uint32_t nrows;
-    uint32_t swap;               // 1: the product's first factor is the higher non-target mode
 };
 
-// Walk one chunk of the stream form.  The chunk's CHUNK entries are loaded
-// once, one 16-byte entry per lane per wave-wide block (CHUNK / 64 blocks,
-// read non-temporally: each is used once): lane L of block b holds position
-// b * EB + L / NG of lane group L % NG, with EB = 64 / NG positions per block.
-// A lane group takes position p's entry from lane (p % EB) * NG + g of its
-// block with ds_bpermute, so entries cost one vector-memory instruction per
-// EB positions instead of one per position (the vector-memory data path is
-// this kernel's bound).  Each lane group runs through its slice (PMAX
-// positions, padded with dummies) in straight-line code, gathering D
-// positions ahead, so every wait count is exact.  Runs of equal rows are
-// summed in registers and added to the LDS tile once per slice.
-template <int V, int G, int D, typename FT>
-__device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__restrict__ ent, uint32_t swap,
-                                            float *tile, uint32_t tp, int lane) {
+// Walk one chunk of the stream form.  The chunk's entries are loaded once,
+// one 16-byte word per lane per wave-wide block (NP * CHUNK / 64 blocks, read
+// non-temporally: each is used once): lane L of block b of a plane holds
+// position b * EB + L / NG of lane group L % NG, with EB = 64 / NG positions
+// per block.  A lane group takes a word of position p from lane (p % EB) * NG
+// + g with ds_bpermute, so entries cost NP vector-memory instructions per EB
+// positions instead of one or more per position (the vector-memory data path
+// is this kernel's bound).  Each lane group runs through its slice (PMAX
+// positions, padded with dummies) in straight-line code, gathering the N - 1
+// factor rows D positions ahead, so every wait count is exact.  The product
+// is formed in ascending mode order.  Runs of equal rows are summed in
+// registers and added to the LDS tile once per slice.
+template <int N, int V, int G, int D, typename FT>
+__device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__restrict__ ent, float *tile,
+                                            uint32_t tp, int lane) {
     using VT = typename Vec<FT, V>::T;
+    constexpr int K = N - 1;
+    constexpr int NP = StreamShape<N>::NP;
     constexpr int NG = 64 / G;
     constexpr int PMAX = CHUNK / NG;
     constexpr int EB = 64 / NG;
@@ -570,29 +581,31 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
     const uint32_t col = (uint32_t)(lane % G) * V;
     // Uniform bases and 32-bit byte offsets (the host checks that every
     // factor matrix spans < 4 GiB), so each load takes one offset register.
-    const char *f0 = static_cast<const char *>(a.fac[a.order[1]]);
-    const char *f1 = static_cast<const char *>(a.fac[a.order[2]]);
+    const char *f[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) f[i] = static_cast<const char *>(a.fac[(uint32_t)i < a.mode ? i : i + 1]);
     const uint32_t rowb = a.stride * (uint32_t)sizeof(FT);
     const uint32_t colb = (col < a.R ? col : 0u) * (uint32_t)sizeof(FT);
     typedef unsigned v4u __attribute__((ext_vector_type(4)));
     const v4u *eb = reinterpret_cast<const v4u *>(ent) + lane;
-    v4u blk[NB];
+    v4u blk[NP][NB];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) blk[b] = __builtin_nontemporal_load(eb + b * 64);
+    for (int q = 0; q < NP; ++q)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) blk[q][b] = __builtin_nontemporal_load(eb + q * CHUNK + b * 64);
     const int gsrc = (int)g * 4;
-    auto part = [&](int p, int w) -> uint32_t {
-        return (uint32_t)__builtin_amdgcn_ds_bpermute(gsrc + (p % EB) * NG * 4, (int)blk[p / EB][w]);
+    auto word = [&](int p, int w) -> uint32_t {
+        return (uint32_t)__builtin_amdgcn_ds_bpermute(gsrc + (p % EB) * NG * 4, (int)blk[w / 4][p / EB][w % 4]);
     };
-    VT raw[D][2];
+    VT raw[D][K];
     float vv[D];
     uint32_t rr[D];
     auto issue = [&](int p) {
-        const uint32_t c0 = part(p, 0), c1 = part(p, 1);
-        vv[p % D] = __uint_as_float(part(p, 2));
-        rr[p % D] = part(p, 3);
-        const uint32_t i0 = swap ? c1 : c0, i1 = swap ? c0 : c1;
-        raw[p % D][0] = *reinterpret_cast<const VT *>(f0 + (uint64_t)(i0 * rowb + colb));
-        raw[p % D][1] = *reinterpret_cast<const VT *>(f1 + (uint64_t)(i1 * rowb + colb));
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            raw[p % D][i] = *reinterpret_cast<const VT *>(f[i] + (uint64_t)(word(p, i) * rowb + colb));
+        vv[p % D] = __uint_as_float(word(p, K));
+        rr[p % D] = word(p, K + 1);
     };
 #pragma unroll
     for (int p = 0; p < D && p < PMAX; ++p) issue(p);
@@ -607,9 +620,12 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
         const float v = vv[p % D];
 #pragma unroll
         for (int k = 0; k < V; ++k) prod[k] *= v;
-        to_float(raw[p % D][1], t);
 #pragma unroll
-        for (int k = 0; k < V; ++k) prod[k] *= t[k];
+        for (int i = 1; i < K; ++i) {
+            to_float(raw[p % D][i], t);
+#pragma unroll
+            for (int k = 0; k < V; ++k) prod[k] *= t[k];
+        }
         const uint32_t row = rr[p % D];
         if (p + D < PMAX) issue(p + D);
         const bool brk = row != cur;
@@ -629,26 +645,29 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
     }
 }
 
-// Pipeline of the stream walk: gather depth D and the minimum resident waves
-// per SIMD the register budget is sized for.  Measured on the 100M-nonzero
-// order-3 Zipf tensor at R = 32 (FP32): D = 4 at five waves per SIMD (111
-// VGPRs) was the fastest of D = 2..8 at two to eight waves; deeper rings at
-// fewer waves were up to 10% slower.
-template <int V, typename FT>
+// Pipeline of the stream walk: gather depth D (about 32 registers of rows in
+// flight) and the minimum resident waves per SIMD the register budget is
+// sized for.  Measured on the 100M-nonzero order-3 Zipf tensor at R = 32
+// (FP32): D = 4 at five waves per SIMD (111 VGPRs) was the fastest of D =
+// 2..8 at two to eight waves; deeper rings at fewer waves were up to 10%
+// slower.
+template <int N, int V, typename FT>
 struct StreamConfig {
-    static constexpr int D = 4;
-    static constexpr int OCC = sizeof(FT) * V >= 16 ? 5 : 6;
+    static constexpr int ROW_REGS = (N - 1) * V * (int)sizeof(FT) / 4;
+    static constexpr int D = ROW_REGS <= 8 ? 4 : (ROW_REGS <= 16 ? 2 : 1);
+    static constexpr int OCC = N > 3 ? 4 : ((int)sizeof(FT) * V >= 16 ? 5 : 6);
 };
 
-// Order-3 HiCOO MTTKRP on the stream form: one workgroup per unit, W waves
-// taking the unit's chunks in turn, an LDS tile of the block-row's output
-// rows written back once (stores when the unit owns the block-row, else
-// atomics).  The factor gathers address a uniform base with 32-bit byte
-// offsets: the host sends factor matrices of 4 GiB or more to the tile
-// kernel.
-template <int V, int G, typename FT, int W>
-__global__ void __launch_bounds__(W * 64, (StreamConfig<V, FT>::OCC)) hicoo_stream_kernel(StreamArgs h, FactorArgs a) {
-    constexpr int D = StreamConfig<V, FT>::D;
+// HiCOO MTTKRP on the stream form: one workgroup per unit, W waves taking the
+// unit's chunks in turn, an LDS tile of the block-row's output rows written
+// back once (stores when the unit owns the block-row, else atomics).  The
+// factor gathers address a uniform base with 32-bit byte offsets: the host
+// sends factor matrices of 4 GiB or more to the tile kernel.
+template <int N, int V, int G, typename FT, int W>
+__global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, FT>::OCC))
+    hicoo_stream_kernel(StreamArgs h, FactorArgs a) {
+    constexpr int D = StreamConfig<N, V, FT>::D;
+    constexpr int NP = StreamShape<N>::NP;
     extern __shared__ float tile[];  // (1 << sb_bits) rows x TP floats
     constexpr int GV = G * V;
     constexpr int TP = GV + 1;
@@ -661,7 +680,7 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<V, FT>::OCC)) hicoo_stre
     const uint32_t c0 = h.unit_chunk[u], c1 = h.unit_chunk[u + 1];
     const uint32_t w0 = __builtin_amdgcn_readfirstlane(wid);
     for (uint32_t c = c0 + w0; c < c1; c += W)
-        stream_walk<V, G, D, FT>(a, h.ent + (uint64_t)c * CHUNK, h.swap, tile, TP, lane);
+        stream_walk<N, V, G, D, FT>(a, h.ent + (uint64_t)c * NP * CHUNK, tile, TP, lane);
     __syncthreads();
     const uint32_t base = h.unit_br[u] << h.sb_bits;
     const bool excl = h.unit_excl[u] != 0;
@@ -678,7 +697,6 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<V, FT>::OCC)) hicoo_stre
             unsafeAtomicAdd(dst, v);
     }
 }
-
 
 template <int N, int V, int G, typename FT>
 __global__ void __launch_bounds__(256) coo_mttkrp_kernel(COOArgs c, FactorArgs a) {
