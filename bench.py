@@ -20,12 +20,16 @@ bounded sample of the same tensor.  roofline.traffic is the HBM bytes per
 launch from the committed rocprofv3 counter summary of this kernel (null if
 that summary was taken on another kernel version).
 
+The "coo" object is ptiMTTKRP's COO kernel on the same tensor and factors
+(ms per step, its roofline with the COO byte convention, and the reference's
+ptiOmpMTTKRP timed on a sample of the same tensor).
+
 The "cpd" object is the CPD-ALS leg: an order-5 tensor 1M x 500k x 100k x
 1k x 24 with 150M nonzeros, rank 16, 20 iterations on the device from one
 seeded set of factors, all-FP32 versus the BF16-then-FP32 schedule (s/iter
 and final fit of each).
 
-    python bench.py [--steps K] [--warmup W] [--nnz NNZ] [--no-cpu] [--no-cpd]
+    python bench.py [--steps K] [--warmup W] [--nnz NNZ] [--no-cpu] [--no-cpd] [--no-coo]
 """
 import argparse
 import ctypes as C
@@ -55,6 +59,7 @@ def parse():
     p.add_argument("--no-bf16", action="store_true")
     p.add_argument("--cpu-sample-nnz", type=float, default=8e6)
     p.add_argument("--no-cpd", action="store_true", help="skip the CPD-ALS leg")
+    p.add_argument("--no-coo", action="store_true", help="skip the COO (ptiMTTKRP) leg")
     p.add_argument("--cpd-nnz", type=float, default=150e6)
     p.add_argument("--cpd-rank", type=int, default=16)
     p.add_argument("--cpd-iters", type=int, default=20)
@@ -181,6 +186,95 @@ def cpu_baseline(h, ndims, facs_np, R, sample_nnz, nthreads):
     return total / z_end * int(a.nnz) * 1e3, kind, z_end, k_end
 
 
+def coo_leg(X, ndims, dfacs, facs_np, outs, R, args, nthreads):
+    """ptiMTTKRP's kernel (COO, per-mode copies sorted by the output index)
+    on the same tensor and factors: ms per three-mode step with HIP events,
+    its roofline with the reference's COO convention B = nnz*(N*4 + 4) +
+    sum_m ndims[m]*R*4 (benchmark/tensor/mttkrp.c:304-307), and the CPU
+    baseline: the reference's own ptiOmpMTTKRP (src/sptensor/mttkrp_omp.c:
+    125-205, nonzero-parallel with atomics) on the first cpu_sample_nnz
+    nonzeros, extrapolated per nonzero (the oracle's restatement where the
+    reference library is absent)."""
+    import torch
+    from hiparti_amd import _ffi as F
+    from hiparti_amd.device import DevCOO
+    N = len(ndims)
+    nnz = X.nnz()
+    t0 = time.time()
+    Dc = DevCOO(X)
+    for m in range(N):
+        Dc.prepare(m)
+    torch.cuda.synchronize()
+    log("COO upload and per-mode sorts in %.1f s" % (time.time() - t0))
+
+    def step(events=None):
+        for mode in range(N):
+            if events is not None:
+                events[mode][0].record()
+            Dc.mttkrp(dfacs, mode, R=R, out=outs[mode])
+            if events is not None:
+                events[mode][1].record()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(N)]
+          for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(events=ev[k])
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    per_mode = [float(np.mean([ev[k][m][0].elapsed_time(ev[k][m][1]) for k in range(args.steps)])) for m in range(N)]
+    del Dc
+    torch.cuda.empty_cache()
+    alg = nnz * (N * 4 + 4) + sum(d * R * 4 for d in ndims)
+    achieved = alg / (np.mean(per_mode) * 1e-3) / 1e9
+    res = {"ms_per_step": round(ms, 4), "per_mode_ms": [round(x, 4) for x in per_mode],
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "kernel": "coo_mttkrp_kernel<3, 4, 8, float> (+ output zero fill)",
+                        "alg_bytes_per_launch": int(alg)}}
+    if not args.no_cpu:
+        t0 = time.time()
+        z = int(min(args.cpu_sample_nnz, nnz))
+        if os.path.exists(REF_LIB):
+            L = C.CDLL(REF_LIB)
+            L.ptiOmpMTTKRP.argtypes = [C.POINTER(F.ptiSparseTensor), C.POINTER(C.POINTER(F.ptiMatrix)),
+                                       C.POINTER(F.ptiIndex), F.ptiIndex, C.c_int]
+            L.ptiOmpMTTKRP.restype = C.c_int
+            view = F.ptiSparseTensor.from_buffer_copy(X.address)   # the first z nonzeros
+            view.nnz = z
+            mats = [F.ptiMatrix(f.shape[0], f.shape[1], f.shape[0], f.shape[1],
+                                f.ctypes.data_as(C.POINTER(F.ptiValue))) for f in facs_np]
+            out = np.zeros((max(ndims), R), np.float32)
+            mats.append(F.ptiMatrix(max(ndims), R, max(ndims), R, out.ctypes.data_as(C.POINTER(F.ptiValue))))
+            ptrs = (C.POINTER(F.ptiMatrix) * (N + 1))(*[C.pointer(m) for m in mats])
+
+            def run(mode):
+                order = (F.ptiIndex * N)(*([mode] + [(mode + i) % N for i in range(1, N)]))
+                mats[N].nrows = ndims[mode]
+                assert L.ptiOmpMTTKRP(C.byref(view), ptrs, order, mode, int(nthreads)) == 0
+            kind, who = "reference", "reference ptiOmpMTTKRP"
+        else:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as O
+            inds = X.inds()[:, :z].copy()
+            vals = X.values()[:z].copy()
+
+            def run(mode):
+                O.mttkrp(inds, vals, facs_np, mode, kind="omp", nthreads=nthreads)
+            kind, who = "port", "oracle restatement of ptiOmpMTTKRP"
+        total = _time_modes(run, N)
+        res["cpu_baseline"] = {"value": round(total / z * nnz * 1e3, 1),
+                               "unit": "ms per step (3-mode COO MTTKRP, extrapolated per nonzero)",
+                               "cores": nthreads, "kind": kind,
+                               "sample": "%s (tk=%d) on the first %d nonzeros of the same tensor, all 3 modes, "
+                                         "1 warm-up + 2 timed calls per mode" % (who, nthreads, z)}
+        log("coo cpu baseline %.1f s" % (time.time() - t0))
+    return {"coo": res}
+
+
 def cpd_leg(args):
     """CP-ALS on the device: order-5 tensor 1M x 500k x 100k x 1k x 24,
     Zipf(1.1) indices on modes 0-2 and uniform on modes 3-4, values U(0,1),
@@ -257,7 +351,6 @@ def main():
     H = X.toHiCOO(7, 14, 14, nthreads=nthreads, device=True)
     nb = H.nblocks()
     log("HiCOO conversion (GPU ordering): nb=%d in %.1f s" % (nb, time.time() - t0))
-    X.free()
     t0 = time.time()
     D = DevHiCOO(H)
     torch.cuda.synchronize()
@@ -331,6 +424,9 @@ def main():
         extra["bf16_per_mode_ms"] = [round(x, 4) for x in bf16_per_mode]
         extra["bf16_hbm_frac"] = round(bf_bytes / (np.mean(bf16_per_mode) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
 
+    if not args.no_coo:
+        extra.update(coo_leg(X, ndims, dfacs, facs_np, outs, R, args, nthreads))
+    X.free()
     cpu = None
     cpu_args = (H, ndims, facs_np, R)
     if not args.no_cpd:
