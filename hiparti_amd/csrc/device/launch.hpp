@@ -45,6 +45,7 @@ constexpr bool stream_supported(uint32_t N, int G) { return N <= 5 && G <= 16; }
     hipError_t launch_hicoo_##N(const HiCOOArgs &h, const FactorArgs &a, Shape sh, bool bf16, hipStream_t s); \
     hipError_t launch_stream_##N(const StreamArgs &h, const FactorArgs &a, Shape sh, bool bf16, hipStream_t s); \
     hipError_t launch_coo_##N(const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s);        \
+    hipError_t launch_coo_stream_##N(const CooStreamArgs &h, const FactorArgs &a, Shape sh, hipStream_t s); \
     hipError_t launch_completion_##N(const COOArgs &c, const FactorArgs &a, Shape sh, float *resid,    \
                                      double *partials, hipStream_t s);
 
@@ -83,6 +84,13 @@ inline hipError_t launch_stream(uint32_t N, const StreamArgs &h, const FactorArg
 }
 inline hipError_t launch_coo(uint32_t N, const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s) {
 #define PTI_CALL(n) launch_coo_##n(c, a, sh, s)
+    PTI_DISPATCH_N(N, PTI_CALL)
+#undef PTI_CALL
+}
+
+inline hipError_t launch_coo_stream(uint32_t N, const CooStreamArgs &h, const FactorArgs &a, Shape sh,
+                                    hipStream_t s) {
+#define PTI_CALL(n) launch_coo_stream_##n(h, a, sh, s)
     PTI_DISPATCH_N(N, PTI_CALL)
 #undef PTI_CALL
 }

@@ -33,7 +33,10 @@
 //
 // COO kernel (coo_mttkrp_kernel).  Per target mode the device keeps a copy
 // sorted by the output index; each run of equal rows in a chunk is added to
-// the output with one FP32 atomic per rank.
+// the output with one FP32 atomic per rank.  coo_stream_kernel walks the
+// same copy in its stream form (decoded entries, as above) and is used where
+// the stream kernel's shapes allow; coo_mttkrp_kernel serves the rest and
+// the replaced-values form (CP-completion residuals).
 //
 // Run reduction (COO kernel).  Within a step the NG contributions are
 // combined by a segmented inclusive scan over groups keyed on the output row;
@@ -567,9 +570,38 @@ struct StreamArgs {
 // factor rows D positions ahead, so every wait count is exact.  The product
 // is formed in ascending mode order.  Runs of equal rows are summed in
 // registers and added to the LDS tile once per slice.
-template <int N, int V, int G, int D, typename FT>
-__device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__restrict__ ent, float *tile,
-                                            uint32_t tp, int lane) {
+// Where a slice's row runs go.  TileSink: the workgroup's LDS tile of the
+// block-row (HiCOO), indexed by the row inside the block-row; lanes past the
+// rank add into tile columns that are never written out.  GlobalRowSink: the
+// output matrix with FP32 atomics (COO), lanes past the rank add nothing.
+struct TileSink {
+    static constexpr uint32_t DUMMY = DUMMY_ROW;
+    float *tile;
+    uint32_t tp;
+    template <int V>
+    __device__ __forceinline__ void add(uint32_t row, uint32_t col, const float *v) const {
+        float *dst = tile + row * tp + col;
+#pragma unroll
+        for (int k = 0; k < V; ++k) atomicAdd(dst + k, v[k]);
+    }
+};
+struct GlobalRowSink {
+    static constexpr uint32_t DUMMY = INVALID_ROW;
+    float *out;
+    uint32_t out_stride;
+    uint32_t R;
+    template <int V>
+    __device__ __forceinline__ void add(uint32_t row, uint32_t col, const float *v) const {
+        if (col >= R) return;
+        float *dst = out + (uint64_t)row * out_stride + col;
+#pragma unroll
+        for (int k = 0; k < V; ++k) unsafeAtomicAdd(dst + k, v[k]);
+    }
+};
+
+template <int N, int V, int G, int D, typename FT, typename Sink>
+__device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__restrict__ ent, const Sink &sink,
+                                            int lane) {
     using VT = typename Vec<FT, V>::T;
     constexpr int K = N - 1;
     constexpr int NP = StreamShape<N>::NP;
@@ -612,7 +644,7 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
     float acc[V];
 #pragma unroll
     for (int k = 0; k < V; ++k) acc[k] = 0.f;
-    uint32_t cur = DUMMY_ROW;
+    uint32_t cur = Sink::DUMMY;
 #pragma unroll
     for (int p = 0; p < PMAX; ++p) {
         float prod[V], t[V];
@@ -629,20 +661,12 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
         const uint32_t row = rr[p % D];
         if (p + D < PMAX) issue(p + D);
         const bool brk = row != cur;
-        if (brk && cur != DUMMY_ROW) {
-            float *dst = tile + cur * tp + col;
-#pragma unroll
-            for (int k = 0; k < V; ++k) atomicAdd(dst + k, acc[k]);
-        }
+        if (brk && cur != Sink::DUMMY) sink.template add<V>(cur, col, acc);
 #pragma unroll
         for (int k = 0; k < V; ++k) acc[k] = brk ? prod[k] : acc[k] + prod[k];
         cur = row;
     }
-    if (cur != DUMMY_ROW) {
-        float *dst = tile + cur * tp + col;
-#pragma unroll
-        for (int k = 0; k < V; ++k) atomicAdd(dst + k, acc[k]);
-    }
+    if (cur != Sink::DUMMY) sink.template add<V>(cur, col, acc);
 }
 
 // Pipeline of the stream walk: gather depth D (about 32 registers of rows in
@@ -680,7 +704,7 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, FT>::OCC))
     const uint32_t c0 = h.unit_chunk[u], c1 = h.unit_chunk[u + 1];
     const uint32_t w0 = __builtin_amdgcn_readfirstlane(wid);
     for (uint32_t c = c0 + w0; c < c1; c += W)
-        stream_walk<N, V, G, D, FT>(a, h.ent + (uint64_t)c * NP * CHUNK, tile, TP, lane);
+        stream_walk<N, V, G, D, FT>(a, h.ent + (uint64_t)c * NP * CHUNK, TileSink{tile, TP}, lane);
     __syncthreads();
     const uint32_t base = h.unit_br[u] << h.sb_bits;
     const bool excl = h.unit_excl[u] != 0;
@@ -696,6 +720,28 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, FT>::OCC))
         else
             unsafeAtomicAdd(dst, v);
     }
+}
+
+// COO MTTKRP on the stream form of a per-mode sorted copy: chunk c holds
+// nonzeros [c * CHUNK, (c + 1) * CHUNK) of the copy in the StreamShape layout,
+// with the full output row as the last word (INVALID_ROW for padding).  One
+// wave per chunk, four per workgroup; row runs go to the output with FP32
+// atomics, one per run per lane-group slice.
+struct CooStreamArgs {
+    const uint4 *ent;
+    uint32_t nchunks;
+};
+
+template <int N, int V, int G, typename FT>
+__global__ void __launch_bounds__(256, (StreamConfig<N, V, FT>::OCC)) coo_stream_kernel(CooStreamArgs h, FactorArgs a) {
+    constexpr int D = StreamConfig<N, V, FT>::D;
+    constexpr int NP = StreamShape<N>::NP;
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t chunk = remap_block(blockIdx.x, gridDim.x) * WAVES_PER_BLOCK +
+                           (uint32_t)__builtin_amdgcn_readfirstlane(wid);
+    if (chunk >= h.nchunks) return;   // whole wave
+    stream_walk<N, V, G, D, FT>(a, h.ent + (uint64_t)chunk * NP * CHUNK, GlobalRowSink{a.out, a.out_stride, a.R},
+                                lane);
 }
 
 template <int N, int V, int G, typename FT>
