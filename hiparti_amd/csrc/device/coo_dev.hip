@@ -236,7 +236,7 @@ int run_coo(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, con
     c.nchunks = (uint32_t)((t->nnz + CHUNK - 1) / CHUNK);
     // The stream kernel takes ranks up to 64 (one launch) when its shapes
     // allow and every gathered factor matrix spans < 4 GiB (32-bit offsets).
-    if (!vals_override && R <= 64) {
+    if (false && !vals_override && R <= 64) {
         Shape sh;
         bool ok = choose_shape(R, stride, stride, sh) && stream_supported(N, sh.G);
         for (uint32_t m = 0; m < N && ok; ++m)
