@@ -6,8 +6,10 @@
 // pairs with rocPRIM, then a gather), so that equal output rows are adjacent
 // and the kernel's segmented reduction collapses them into one atomic per
 // (chunk, row) run.  Where the stream kernel's shapes allow, the sorted copy
-// also gets a stream form on its first MTTKRP (decoded entries in the layout
-// of StreamShape, 16 bytes per nonzero and plane), read by coo_stream_kernel.
+// is instead cut into blocks of 2^7 rows, chunks and units like a HiCOO mode
+// copy and gets a stream form on its first MTTKRP (decoded entries in the
+// layout of StreamShape, 16 bytes per nonzero and plane), which
+// hicoo_stream_kernel runs with its LDS tile per block.
 #include <atomic>
 #include <mutex>
 #include <unordered_map>
@@ -26,6 +28,7 @@ struct ptiDevCOO {
     pti::dev::Buffer sinds[8][8], svals[8];   // [target mode][mode] sorted copies
     pti::dev::Buffer perm[8];                 // [target mode] sorted position -> storage position
     pti::dev::Buffer partials;                // completion forward: per-workgroup sums
+    pti::dev::RowUnits units[8];              // [target mode] row blocks / chunks / units of the sorted copy
     pti::dev::Buffer stream[8];               // [target mode] stream form of the sorted copy
     uint32_t stream_ng[8] = {0};              // its lane-group count (0: not built)
     bool ready[8] = {false};
@@ -69,30 +72,31 @@ struct Ptr8 {
     const uint32_t *p[8];
 };
 
-// Stream form of the copy sorted by mode `mode` (one wave per chunk of CHUNK
-// consecutive nonzeros): entries in the StreamShape layout for ng lane groups,
-// the coordinates of the other modes in ascending mode order, the value bits
-// and the full output row (INVALID_ROW for padding).
-__global__ void __launch_bounds__(256) coo_stream_fill_kernel(Ptr8 sinds, const float *__restrict__ svals, uint64_t nnz,
-                                                              uint32_t nchunks, uint32_t N, uint32_t mode,
-                                                              uint32_t ng, uint4 *__restrict__ ent) {
+// Stream form of the copy sorted by mode `mode`, cut by build_row_units into
+// blocks of 2^ROW_BITS rows, chunks and units (one wave per chunk): entries in
+// the StreamShape layout for ng lane groups, the coordinates of the other
+// modes in ascending mode order, the value bits and the row inside its block.
+constexpr uint32_t ROW_BITS = 7;
+__global__ void __launch_bounds__(256) coo_stream_fill_kernel(Ptr8 sinds, const float *__restrict__ svals,
+                                                              const uint32_t *__restrict__ chunk_z, uint32_t nchunks,
+                                                              uint32_t N, uint32_t mode, uint32_t ng,
+                                                              uint4 *__restrict__ ent) {
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t c = blockIdx.x * 4 + wid;
     if (c >= nchunks) return;   // whole wave
     const uint32_t NP = (N + 4) / 4;
-    const uint64_t z0 = (uint64_t)c * CHUNK;
-    const uint32_t cnt = (uint32_t)min<uint64_t>((uint64_t)CHUNK, nnz - z0);
+    const uint32_t z0 = chunk_z[c], cnt = chunk_z[c + 1] - z0;
     const uint32_t per = (cnt + ng - 1) / ng;
     for (uint32_t e = lane; e < CHUNK; e += 64) {
         const uint32_t g = e % ng, j = e / ng;
         uint32_t w[12];
         for (uint32_t k = 0; k < 12; ++k) w[k] = 0u;
-        w[N] = INVALID_ROW;
+        w[N] = DUMMY_ROW;
         if (j < per && g * per + j < cnt) {
-            const uint64_t z = z0 + g * per + j;
+            const uint64_t z = (uint64_t)z0 + g * per + j;
             for (uint32_t i = 0; i + 1 < N; ++i) w[i] = sinds.p[i < mode ? i : i + 1][z];
             w[N - 1] = __float_as_uint(svals[z]);
-            w[N] = sinds.p[mode][z];
+            w[N] = sinds.p[mode][z] & ((1u << ROW_BITS) - 1u);
         }
         for (uint32_t q = 0; q < NP; ++q)
             ent[((uint64_t)c * NP + q) * CHUNK + e] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
@@ -203,12 +207,14 @@ int ensure_coo_stream(ptiDevCOO *t, ptiIndex mode, uint32_t ng, hipStream_t s, c
     if (t->stream[mode].ptr) PTI_HIP(hipDeviceSynchronize(), M);
     const uint32_t N = t->nmodes;
     const uint32_t NP = (N + 4) / 4;
-    const uint32_t nchunks = (uint32_t)((t->nnz + CHUNK - 1) / CHUNK);
-    PTI_HIP(t->stream[mode].alloc((size_t)nchunks * NP * CHUNK * sizeof(uint4)), M);
+    RowUnits &u = t->units[mode];
+    if (!u.chunk_z.ptr) PTI_CHECK(build_row_units(t->sinds[mode][mode].as<uint32_t>(), t->nnz, ROW_BITS, u, s, M), M);
+    PTI_HIP(t->stream[mode].alloc((size_t)u.nchunks * NP * CHUNK * sizeof(uint4)), M);
     Ptr8 p{};
     for (uint32_t m = 0; m < N; ++m) p.p[m] = t->sinds[mode][m].as<uint32_t>();
-    hipLaunchKernelGGL(coo_stream_fill_kernel, dim3((nchunks + 3) / 4), dim3(256), 0, s, p, t->svals[mode].as<float>(),
-                       t->nnz, nchunks, N, (uint32_t)mode, ng, t->stream[mode].as<uint4>());
+    hipLaunchKernelGGL(coo_stream_fill_kernel, dim3((u.nchunks + 3) / 4), dim3(256), 0, s, p,
+                       t->svals[mode].as<float>(), u.chunk_z.as<uint32_t>(), u.nchunks, N, (uint32_t)mode, ng,
+                       t->stream[mode].as<uint4>());
     PTI_HIP(hipGetLastError(), M);
     t->stream_ng[mode] = ng;
     return 0;
@@ -236,7 +242,7 @@ int run_coo(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, con
     c.nchunks = (uint32_t)((t->nnz + CHUNK - 1) / CHUNK);
     // The stream kernel takes ranks up to 64 (one launch) when its shapes
     // allow and every gathered factor matrix spans < 4 GiB (32-bit offsets).
-    if (false && !vals_override && R <= 64) {
+    if (!vals_override && R <= 64) {
         Shape sh;
         bool ok = choose_shape(R, stride, stride, sh) && stream_supported(N, sh.G);
         for (uint32_t m = 0; m < N && ok; ++m)
@@ -251,10 +257,16 @@ int run_coo(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, con
             a.out_stride = stride;
             a.mode = mode;
             for (uint32_t i = 0; i < N; ++i) a.order[i] = mats_order[i];
-            CooStreamArgs h;
+            const RowUnits &u = t->units[mode];
+            StreamArgs h;
             h.ent = t->stream[mode].as<uint4>();
-            h.nchunks = c.nchunks;
-            PTI_HIP(launch_coo_stream(N, h, a, sh, s), M);
+            h.unit_chunk = u.unit_chunk.as<uint32_t>();
+            h.unit_br = u.unit_br.as<uint32_t>();
+            h.unit_excl = u.unit_excl.as<uint32_t>();
+            h.nunits = u.nunits;
+            h.sb_bits = ROW_BITS;
+            h.nrows = t->ndims[mode];
+            PTI_HIP(launch_stream(N, h, a, sh, false, s), M);
             return 0;
         }
     }

@@ -364,6 +364,26 @@ __global__ void __launch_bounds__(256) stream_fill_kernel(HiCOOArgs h, uint32_t 
     }
 }
 
+// Row blocks of nonzeros sorted by output row: flag[z] = 1 where row >> sb
+// changes.
+__global__ void __launch_bounds__(256) row_block_flag_kernel(const uint32_t *__restrict__ rows, uint64_t n, uint32_t sb,
+                                                             uint32_t *__restrict__ flag) {
+    for (uint64_t z = (uint64_t)blockIdx.x * 256 + threadIdx.x; z < n; z += (uint64_t)gridDim.x * 256)
+        flag[z] = (z == 0 || (rows[z] >> sb) != (rows[z - 1] >> sb)) ? 1u : 0u;
+}
+
+// Block s starts at nonzero start[s] and holds rows key[s] << sb ...
+__global__ void __launch_bounds__(256) row_block_start_kernel(const uint32_t *__restrict__ rows, uint64_t n,
+                                                              uint32_t sb, const uint32_t *__restrict__ flag,
+                                                              const uint32_t *__restrict__ segid,
+                                                              uint32_t *__restrict__ start, uint32_t *__restrict__ key) {
+    for (uint64_t z = (uint64_t)blockIdx.x * 256 + threadIdx.x; z < n; z += (uint64_t)gridDim.x * 256)
+        if (flag[z]) {
+            start[segid[z] - 1] = (uint32_t)z;
+            key[segid[z] - 1] = rows[z] >> sb;
+        }
+}
+
 template <typename T>
 int dev_put(Buffer &b, const T *src, size_t n, hipStream_t s, const char *M) {
     PTI_HIP(b.alloc(n * sizeof(T)), M);
@@ -686,6 +706,88 @@ void hicoo_view(const ptiDevHiCOO *t, uint32_t *nmodes, const ptiIndex **ndims, 
     *vals = nullptr;
     for (uint32_t m = 0; m < t->nmodes && !*vals; ++m)
         if (t->mc[m].ready) *vals = t->mc[m].vals.as<float>();
+}
+
+// Cuts n nonzeros sorted by output row (rows[]) into blocks of 2^sb rows,
+// chunks of <= CHUNK nonzeros that never cross a block and units of <=
+// UNIT_CHUNKS chunks of one block, the structure of a HiCOO mode copy (so the
+// stream kernel can run a sorted COO copy).
+int build_row_units(const uint32_t *rows, uint64_t n, uint32_t sb, RowUnits &u, hipStream_t s, const char *M) {
+    u.nchunks = u.nunits = 0;
+    if (n == 0) return 0;
+    Buffer flag, segid, tmp_scan;
+    PTI_HIP(flag.alloc(n * 4), M);
+    PTI_HIP(segid.alloc(n * 4), M);
+    hipLaunchKernelGGL(row_block_flag_kernel, grid_n(n), dim3(256), 0, s, rows, n, sb, flag.as<uint32_t>());
+    PTI_HIP(hipGetLastError(), M);
+    size_t tb = 0;
+    PTI_HIP(rocprim::inclusive_scan(nullptr, tb, flag.as<uint32_t>(), segid.as<uint32_t>(), (size_t)n,
+                                    rocprim::plus<uint32_t>(), s),
+            M);
+    PTI_HIP(tmp_scan.alloc(tb), M);
+    PTI_HIP(rocprim::inclusive_scan(tmp_scan.ptr, tb, flag.as<uint32_t>(), segid.as<uint32_t>(), (size_t)n,
+                                    rocprim::plus<uint32_t>(), s),
+            M);
+    uint32_t nseg = 0;
+    PTI_HIP(hipMemcpyAsync(&nseg, segid.as<uint32_t>() + n - 1, 4, hipMemcpyDeviceToHost, s), M);
+    PTI_HIP(hipStreamSynchronize(s), M);
+    Buffer start, key, idx, nch, nu, choff, uoff, tmp_seg, cblk;
+    PTI_HIP(start.alloc((size_t)(nseg + 1) * 4), M);
+    PTI_HIP(key.alloc((size_t)(nseg + 1) * 4), M);
+    PTI_HIP(idx.alloc((size_t)(nseg + 1) * 4), M);
+    hipLaunchKernelGGL(row_block_start_kernel, grid_n(n), dim3(256), 0, s, rows, n, sb, flag.as<uint32_t>(),
+                       segid.as<uint32_t>(), start.as<uint32_t>(), key.as<uint32_t>());
+    PTI_HIP(hipGetLastError(), M);
+    const uint32_t n32 = (uint32_t)n;
+    PTI_HIP(hipMemcpyAsync(start.as<uint32_t>() + nseg, &n32, 4, hipMemcpyHostToDevice, s), M);
+    hipLaunchKernelGGL(iota_u32_kernel, grid_n(nseg + 1), dim3(256), 0, s, idx.as<uint32_t>(), (uint64_t)nseg + 1);
+    PTI_HIP(hipGetLastError(), M);
+    // Segments here are row blocks; each is its own "first block" (idx) and
+    // `start` plays the block offsets.
+    PTI_HIP(nch.alloc((size_t)(nseg + 1) * 4), M);
+    PTI_HIP(nu.alloc((size_t)(nseg + 1) * 4), M);
+    PTI_HIP(choff.alloc((size_t)(nseg + 1) * 4), M);
+    PTI_HIP(uoff.alloc((size_t)(nseg + 1) * 4), M);
+    hipLaunchKernelGGL(segment_counts_kernel, grid_n(nseg), dim3(256), 0, s, idx.as<uint32_t>(), start.as<uint32_t>(),
+                       nseg, UNIT_CHUNKS, nch.as<uint32_t>(), nu.as<uint32_t>());
+    PTI_HIP(hipGetLastError(), M);
+    const uint32_t zero = 0;
+    PTI_HIP(hipMemcpyAsync(nch.as<uint32_t>() + nseg, &zero, 4, hipMemcpyHostToDevice, s), M);
+    PTI_HIP(hipMemcpyAsync(nu.as<uint32_t>() + nseg, &zero, 4, hipMemcpyHostToDevice, s), M);
+    size_t sb3 = 0;
+    PTI_HIP(rocprim::exclusive_scan(nullptr, sb3, nch.as<uint32_t>(), choff.as<uint32_t>(), 0u, (size_t)nseg + 1,
+                                    rocprim::plus<uint32_t>(), s),
+            M);
+    PTI_HIP(tmp_seg.alloc(sb3), M);
+    PTI_HIP(rocprim::exclusive_scan(tmp_seg.ptr, sb3, nch.as<uint32_t>(), choff.as<uint32_t>(), 0u, (size_t)nseg + 1,
+                                    rocprim::plus<uint32_t>(), s),
+            M);
+    PTI_HIP(rocprim::exclusive_scan(tmp_seg.ptr, sb3, nu.as<uint32_t>(), uoff.as<uint32_t>(), 0u, (size_t)nseg + 1,
+                                    rocprim::plus<uint32_t>(), s),
+            M);
+    uint32_t tot[2] = {0, 0};
+    PTI_HIP(hipMemcpyAsync(&tot[0], choff.as<uint32_t>() + nseg, 4, hipMemcpyDeviceToHost, s), M);
+    PTI_HIP(hipMemcpyAsync(&tot[1], uoff.as<uint32_t>() + nseg, 4, hipMemcpyDeviceToHost, s), M);
+    PTI_HIP(hipStreamSynchronize(s), M);
+    u.nchunks = tot[0];
+    u.nunits = tot[1];
+    PTI_HIP(u.chunk_z.alloc((size_t)(u.nchunks + 1) * 4), M);
+    PTI_HIP(cblk.alloc((size_t)u.nchunks * 4), M);
+    PTI_HIP(u.unit_chunk.alloc((size_t)(u.nunits + 1) * 4), M);
+    PTI_HIP(u.unit_br.alloc((size_t)u.nunits * 4), M);
+    PTI_HIP(u.unit_excl.alloc((size_t)u.nunits * 4), M);
+    hipLaunchKernelGGL(chunk_fill_kernel, grid_n(u.nchunks), dim3(256), 0, s, choff.as<uint32_t>(), idx.as<uint32_t>(),
+                       start.as<uint32_t>(), nseg, u.nchunks, u.chunk_z.as<uint32_t>(), cblk.as<uint32_t>());
+    PTI_HIP(hipGetLastError(), M);
+    PTI_HIP(hipMemcpyAsync(u.chunk_z.as<uint32_t>() + u.nchunks, &n32, 4, hipMemcpyHostToDevice, s), M);
+    hipLaunchKernelGGL(unit_fill_kernel, grid_n(u.nunits), dim3(256), 0, s, uoff.as<uint32_t>(), choff.as<uint32_t>(),
+                       idx.as<uint32_t>(), key.as<uint32_t>(), nseg, u.nunits, u.unit_chunk.as<uint32_t>(),
+                       u.unit_br.as<uint32_t>(), u.unit_excl.as<uint32_t>());
+    PTI_HIP(hipGetLastError(), M);
+    PTI_HIP(hipMemcpyAsync(u.unit_chunk.as<uint32_t>() + u.nunits, &u.nchunks, 4, hipMemcpyHostToDevice, s), M);
+    // The host scalars and the temporaries must outlive the queued work.
+    PTI_HIP(hipStreamSynchronize(s), M);
+    return 0;
 }
 
 }  // namespace dev

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "mttkrp_kernels.hpp"
+#include "runtime.hpp"
 
 namespace pti {
 namespace dev {
@@ -36,6 +37,14 @@ inline bool choose_shape(uint32_t R, uint32_t stride, uint32_t out_stride, Shape
     return true;
 }
 
+// A sorted copy's rows cut into 2^sb-row blocks, chunks and units, the
+// structure of a HiCOO mode copy (build_row_units in hicoo_dev.hip).
+struct RowUnits {
+    Buffer chunk_z, unit_chunk, unit_br, unit_excl;
+    uint32_t nchunks = 0, nunits = 0;
+};
+int build_row_units(const uint32_t *rows, uint64_t n, uint32_t sb, RowUnits &u, hipStream_t s, const char *M);
+
 // Shapes the stream kernel runs: orders up to 5 (beyond, the entry planes and
 // the gathered rows no longer fit the register budget) and lane groups of at
 // most 16 lanes (a slice of at most 64 positions, walked straight-line).
@@ -45,7 +54,6 @@ constexpr bool stream_supported(uint32_t N, int G) { return N <= 5 && G <= 16; }
     hipError_t launch_hicoo_##N(const HiCOOArgs &h, const FactorArgs &a, Shape sh, bool bf16, hipStream_t s); \
     hipError_t launch_stream_##N(const StreamArgs &h, const FactorArgs &a, Shape sh, bool bf16, hipStream_t s); \
     hipError_t launch_coo_##N(const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s);        \
-    hipError_t launch_coo_stream_##N(const CooStreamArgs &h, const FactorArgs &a, Shape sh, hipStream_t s); \
     hipError_t launch_completion_##N(const COOArgs &c, const FactorArgs &a, Shape sh, float *resid,    \
                                      double *partials, hipStream_t s);
 
@@ -84,13 +92,6 @@ inline hipError_t launch_stream(uint32_t N, const StreamArgs &h, const FactorArg
 }
 inline hipError_t launch_coo(uint32_t N, const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s) {
 #define PTI_CALL(n) launch_coo_##n(c, a, sh, s)
-    PTI_DISPATCH_N(N, PTI_CALL)
-#undef PTI_CALL
-}
-
-inline hipError_t launch_coo_stream(uint32_t N, const CooStreamArgs &h, const FactorArgs &a, Shape sh,
-                                    hipStream_t s) {
-#define PTI_CALL(n) launch_coo_stream_##n(h, a, sh, s)
     PTI_DISPATCH_N(N, PTI_CALL)
 #undef PTI_CALL
 }

@@ -86,28 +86,6 @@ hipError_t stream_v(const StreamArgs &h, const FactorArgs &a, int G, bool bf16, 
 }
 
 template <int V, int G>
-hipError_t coo_stream_vg(const CooStreamArgs &h, const FactorArgs &a, hipStream_t s) {
-    if constexpr (!stream_supported(N, G)) {
-        return hipErrorInvalidValue;
-    } else {
-        if (h.nchunks == 0) return hipSuccess;
-        hipLaunchKernelGGL((coo_stream_kernel<N, V, G, float>), dim3(grid_for_chunks(h.nchunks)), dim3(256), 0, s, h,
-                           a);
-        return hipGetLastError();
-    }
-}
-
-template <int V>
-hipError_t coo_stream_v(const CooStreamArgs &h, const FactorArgs &a, int G, hipStream_t s) {
-    switch (G) {
-        case 4: return coo_stream_vg<V, 4>(h, a, s);
-        case 8: return coo_stream_vg<V, 8>(h, a, s);
-        case 16: return coo_stream_vg<V, 16>(h, a, s);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-template <int V, int G>
 hipError_t coo_vg(const COOArgs &c, const FactorArgs &a, hipStream_t s) {
     if (c.nchunks == 0) return hipSuccess;
     dim3 grid(grid_for_chunks(c.nchunks)), block(256);
@@ -177,16 +155,6 @@ hipError_t PTI_CAT(launch_stream_, PTI_NMODES)(const StreamArgs &h, const Factor
         case 4: return stream_v<4>(h, a, sh.G, bf16, s);
         case 2: return stream_v<2>(h, a, sh.G, bf16, s);
         case 1: return stream_v<1>(h, a, sh.G, bf16, s);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t PTI_CAT(launch_coo_stream_, PTI_NMODES)(const CooStreamArgs &h, const FactorArgs &a, Shape sh,
-                                                   hipStream_t s) {
-    switch (sh.V) {
-        case 4: return coo_stream_v<4>(h, a, sh.G, s);
-        case 2: return coo_stream_v<2>(h, a, sh.G, s);
-        case 1: return coo_stream_v<1>(h, a, sh.G, s);
         default: return hipErrorInvalidValue;
     }
 }

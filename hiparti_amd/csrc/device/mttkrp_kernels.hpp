@@ -33,10 +33,11 @@
 //
 // COO kernel (coo_mttkrp_kernel).  Per target mode the device keeps a copy
 // sorted by the output index; each run of equal rows in a chunk is added to
-// the output with one FP32 atomic per rank.  coo_stream_kernel walks the
-// same copy in its stream form (decoded entries, as above) and is used where
-// the stream kernel's shapes allow; coo_mttkrp_kernel serves the rest and
-// the replaced-values form (CP-completion residuals).
+// the output with one FP32 atomic per rank.  Where the stream kernel's shapes
+// allow, the sorted copy is instead cut into 2^7-row blocks, chunks and units
+// like a HiCOO mode copy and run by hicoo_stream_kernel (LDS tile per block);
+// coo_mttkrp_kernel serves the rest and the replaced-values form
+// (CP-completion residuals).
 //
 // Run reduction (COO kernel).  Within a step the NG contributions are
 // combined by a segmented inclusive scan over groups keyed on the output row;
@@ -570,10 +571,9 @@ struct StreamArgs {
 // factor rows D positions ahead, so every wait count is exact.  The product
 // is formed in ascending mode order.  Runs of equal rows are summed in
 // registers and added to the LDS tile once per slice.
-// Where a slice's row runs go.  TileSink: the workgroup's LDS tile of the
-// block-row (HiCOO), indexed by the row inside the block-row; lanes past the
-// rank add into tile columns that are never written out.  GlobalRowSink: the
-// output matrix with FP32 atomics (COO), lanes past the rank add nothing.
+// Where a slice's row runs go: the workgroup's LDS tile of the block-row,
+// indexed by the row inside the block-row; lanes past the rank add into tile
+// columns that are never written out.
 struct TileSink {
     static constexpr uint32_t DUMMY = DUMMY_ROW;
     float *tile;
@@ -585,20 +585,6 @@ struct TileSink {
         for (int k = 0; k < V; ++k) atomicAdd(dst + k, v[k]);
     }
 };
-struct GlobalRowSink {
-    static constexpr uint32_t DUMMY = INVALID_ROW;
-    float *out;
-    uint32_t out_stride;
-    uint32_t R;
-    template <int V>
-    __device__ __forceinline__ void add(uint32_t row, uint32_t col, const float *v) const {
-        if (col >= R) return;
-        float *dst = out + (uint64_t)row * out_stride + col;
-#pragma unroll
-        for (int k = 0; k < V; ++k) unsafeAtomicAdd(dst + k, v[k]);
-    }
-};
-
 template <int N, int V, int G, int D, typename FT, typename Sink>
 __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__restrict__ ent, const Sink &sink,
                                             int lane) {
@@ -720,28 +706,6 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, FT>::OCC))
         else
             unsafeAtomicAdd(dst, v);
     }
-}
-
-// COO MTTKRP on the stream form of a per-mode sorted copy: chunk c holds
-// nonzeros [c * CHUNK, (c + 1) * CHUNK) of the copy in the StreamShape layout,
-// with the full output row as the last word (INVALID_ROW for padding).  One
-// wave per chunk, four per workgroup; row runs go to the output with FP32
-// atomics, one per run per lane-group slice.
-struct CooStreamArgs {
-    const uint4 *ent;
-    uint32_t nchunks;
-};
-
-template <int N, int V, int G, typename FT>
-__global__ void __launch_bounds__(256, (StreamConfig<N, V, FT>::OCC)) coo_stream_kernel(CooStreamArgs h, FactorArgs a) {
-    constexpr int D = StreamConfig<N, V, FT>::D;
-    constexpr int NP = StreamShape<N>::NP;
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t chunk = remap_block(blockIdx.x, gridDim.x) * WAVES_PER_BLOCK +
-                           (uint32_t)__builtin_amdgcn_readfirstlane(wid);
-    if (chunk >= h.nchunks) return;   // whole wave
-    stream_walk<N, V, G, D, FT>(a, h.ent + (uint64_t)chunk * NP * CHUNK, GlobalRowSink{a.out, a.out_stride, a.R},
-                                lane);
 }
 
 template <int N, int V, int G, typename FT>
