@@ -187,8 +187,9 @@ def cpu_baseline(h, ndims, facs_np, R, sample_nnz, nthreads):
 
 
 def coo_leg(X, ndims, dfacs, facs_np, outs, R, args, nthreads):
-    """ptiMTTKRP's kernel (COO, per-mode copies sorted by the output index)
-    on the same tensor and factors: ms per three-mode step with HIP events,
+    """ptiMTTKRP on the device (COO, per-mode copies sorted by the output
+    index, cut into 128-row blocks and run by the stream kernel) on the same
+    tensor and factors: ms per three-mode step with HIP events,
     its roofline with the reference's COO convention B = nnz*(N*4 + 4) +
     sum_m ndims[m]*R*4 (benchmark/tensor/mttkrp.c:304-307), and the CPU
     baseline: the reference's own ptiOmpMTTKRP (src/sptensor/mttkrp_omp.c:
@@ -233,7 +234,8 @@ def coo_leg(X, ndims, dfacs, facs_np, outs, R, args, nthreads):
     res = {"ms_per_step": round(ms, 4), "per_mode_ms": [round(x, 4) for x in per_mode],
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                        "kernel": "coo_mttkrp_kernel<3, 4, 8, float> (+ output zero fill)",
+                        "kernel": "hicoo_stream_kernel<3, 4, 8, float, 8> on the COO copy's 128-row blocks "
+                                  "(+ output zero fill)",
                         "alg_bytes_per_launch": int(alg)}}
     if not args.no_cpu:
         t0 = time.time()
