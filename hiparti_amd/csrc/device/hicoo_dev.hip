@@ -657,12 +657,20 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
     // would otherwise push a workgroup past 64 KB of LDS.
     const ptiIndex tile = N >= 6 ? 32 : 64;
     const size_t esz = bf16 ? 2 : 4;
+    Shape first{0, 0};
     for (ptiIndex c0 = 0; c0 < R; c0 += tile) {
         const ptiIndex Rt = std::min<ptiIndex>(tile, R - c0);
         // Column tiles start at multiples of 32 elements, so they keep the
         // alignment the vector width V relies on.
         Shape sh;
         if (!choose_shape(Rt, stride, out_stride, sh)) PTI_FAIL(PTIERR_VALUE_ERROR, M, "unsupported rank tile");
+        // A narrower last tile keeps the first tile's lane shape when its ranks
+        // fit it (idle lanes past the rank), so every tile reads the same
+        // stream form instead of rebuilding it for another lane-group count.
+        if (c0 == 0)
+            first = sh;
+        else if (Rt % first.V == 0 && Rt <= (ptiIndex)(first.G * first.V))
+            sh = first;
         // The kernel stages factor-row offsets as 32-bit counts of V-wide vectors.
         for (uint32_t i = 1; i < N; ++i)
             if ((uint64_t)t->ndims[mats_order[i]] * (stride / sh.V) >= (1ull << 32))

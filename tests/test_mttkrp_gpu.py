@@ -88,6 +88,7 @@ SHAPES = [
     ([300, 200, 100], 6000, 33, 7, 9),
     ([300, 200, 100], 6000, 64, 4, 8),
     ([120, 90, 70], 3000, 100, 3, 5),
+    ([120, 90, 70], 3000, 80, 3, 5),
     ([120, 90, 70], 3000, 300, 3, 5),
     ([40, 30, 20, 10], 4000, 16, 2, 3),
     ([30, 20, 15, 10, 8], 5000, 16, 1, 3),
