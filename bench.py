@@ -82,7 +82,7 @@ def copy_calibration(gib):
     print(json.dumps({"copy_calibration_bytes_per_copy": n * 4, "copies": 3}), flush=True)
 
 
-KERNEL = "hicoo_stream_kernel<4, 8, float, 8>"
+KERNEL = "hicoo_stream_kernel<3, 4, 8, float, 8>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hicoo_stream_kernel_zipf100M_R32_counters.md")
 
 
