@@ -274,3 +274,31 @@ def test_order3_factors_of_4gib_take_the_tile_kernel():
         assert rel_fro(got, want[touched]) <= TOL, mode
         assert float(out.abs().sum().item()) == pytest.approx(float(np.abs(want).sum()), rel=1e-5)
         del out
+
+
+def test_stream_paths_on_extreme_row_distributions(oracle):
+    """The stream kernels' run sums, chunk padding and unit write-out on
+    extreme shapes, HiCOO and COO against the FP64 oracle: every nonzero in
+    one output row (one run across many chunks and non-exclusive units),
+    block-rows of exactly 256 * k nonzeros (no padding), and ranks 4 and 60
+    (idle lanes)."""
+    rng = np.random.default_rng(41)
+    ndims = [300, 2000, 1500]
+    # All 60000 nonzeros share output row 7 of mode 0.
+    nnz = 60000
+    flat = rng.choice(ndims[1] * ndims[2], nnz, replace=False)
+    one_row = np.stack([np.full(nnz, 7), flat // ndims[2], flat % ndims[2]]).astype(np.uint32)
+    # Each of 8 block-rows (sb = 7) holds exactly 512 nonzeros.
+    blocks = []
+    for br in range(8):
+        f = rng.choice(128 * ndims[1] * 10, 512, replace=False)
+        blocks.append(np.stack([br * 128 + f % 128, (f // 128) % ndims[1], f // (128 * ndims[1])]))
+    full = np.concatenate(blocks, axis=1).astype(np.uint32)
+    for name, nd, inds in (("one row", ndims, one_row), ("full chunks", [1024, 2000, 10], full)):
+        vals = rng.random(inds.shape[1]).astype(np.float32)
+        for R in (4, 60, 32):
+            facs = [rng.random((d, R)).astype(np.float32) for d in nd]
+            for mode in range(3):
+                f64 = oracle.mttkrp(inds, vals, facs, mode, kind="f64")
+                assert rel_fro(run_hicoo(nd, inds, vals, facs, mode, 7, 10, 10), f64) <= TOL, (name, R, mode)
+                assert rel_fro(run_coo(nd, inds, vals, facs, mode), f64) <= TOL, (name, R, mode, "coo")
