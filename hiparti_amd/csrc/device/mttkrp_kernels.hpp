@@ -1,12 +1,14 @@
 // MTTKRP kernels for gfx950 (wave64).
 //
 // Common pieces.  Nonzeros are processed in chunks of at most CHUNK = 256
-// consecutive nonzeros; a wavefront decodes a chunk into LDS (coordinates of
-// every mode and the values) and then walks it with the 64 lanes split into
+// consecutive nonzeros, walked by a wavefront with its 64 lanes split into
 // NG = 64/G groups of G lanes: a group handles one nonzero per step and each
 // lane V consecutive ranks (float4 / float2 / float loads of the factor
-// rows, or 4/2/1 bf16 values for the BF16 form).  The contribution
-// v * prod_k A_k[i_k][r] is formed in mats_order order, in FP32.
+// rows, or 4/2/1 bf16 values for the BF16 form).  The tile and COO kernels
+// first decode a chunk into LDS (coordinates and values) and form the
+// contribution v * prod_k A_k[i_k][r] in mats_order order; the stream kernel
+// reads chunks already decoded and forms it in ascending mode order.  All in
+// FP32.
 //
 // HiCOO kernel (hicoo_tile_kernel).  The device keeps, for each target mode
 // m, a copy of the HiCOO blocks grouped by their mode-m block coordinate
