@@ -87,8 +87,9 @@ __global__ void __launch_bounds__(256) coo_stream_fill_kernel(Ptr8 sinds, const 
     const uint32_t NP = (N + 4) / 4;
     const uint32_t z0 = chunk_z[c], cnt = chunk_z[c + 1] - z0;
     const uint32_t per = (cnt + ng - 1) / ng;
+    const uint32_t G = 64 / ng;
     for (uint32_t e = lane; e < CHUNK; e += 64) {
-        const uint32_t g = e % ng, j = e / ng;
+        const uint32_t g = (e % 64) / G, j = (e / 64) * G + e % G;   // e == stream_entry(G, g, j)
         uint32_t w[12];
         for (uint32_t k = 0; k < 12; ++k) w[k] = 0u;
         w[N] = DUMMY_ROW;

@@ -53,6 +53,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace pti {
 namespace dev {
@@ -541,10 +542,15 @@ constexpr size_t hicoo_stage_bytes(int W) {
 // inside the block-row; positions past a slice's end are all-zero with row
 // DUMMY_ROW.  Entries are stored in NP = ceil((N + 1) / 4) planes of 16-byte
 // words.  Chunk c owns NP * CHUNK 16-byte words starting at c * NP * CHUNK,
-// plane q first at offset q * CHUNK: its nonzeros are cut into NG slices of
-// per = ceil(cnt / NG) positions and position j of slice g is entry j * NG +
-// g, so 64 consecutive entries of one plane are one wave-wide 16-byte-per-lane
-// load.
+// plane q first at offset q * CHUNK: its nonzeros are cut into NG = 64 / G
+// slices of per = ceil(cnt / NG) positions (G lanes per slice) and position j
+// of slice g is entry stream_entry(G, g, j) = 64 * (j / G) + G * g + j % G, so
+// 64 consecutive entries of one plane are one wave-wide 16-byte-per-lane load
+// in which each lane group holds G positions of its own slice.
+__host__ __device__ constexpr uint32_t stream_entry(uint32_t G, uint32_t g, uint32_t j) {
+    return 64u * (j / G) + G * g + j % G;
+}
+
 template <int N>
 struct StreamShape {
     static constexpr int WORDS = N + 1;
@@ -561,18 +567,62 @@ struct StreamArgs {
     uint32_t nrows;
 };
 
+// Broadcast, inside each group of G lanes, the word that lane J of the group
+// holds.  Groups of 4, 8 and 16 lanes use DPP moves (VALU work: the LDS pipe,
+// which also serves the tile's atomics, stays free).  4 lanes: a quad_perm
+// copies lane J of each quad to the whole quad.  8 lanes: the quad_perm, then
+// a 4-lane row shift written only to the group's other quad (bank mask)
+// copies quad J / 4 there.  16 lanes: row_newbcast copies lane J of each
+// 16-lane row to the whole row.  Wider groups use ds_bpermute.
+template <int G, int J>
+__device__ __forceinline__ uint32_t group_bcast(uint32_t x) {
+    static_assert(G == 4 || G == 8 || G == 16, "DPP broadcast covers groups of 4, 8 and 16 lanes");
+    if constexpr (G == 16) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x150 + J, 0xF, 0xF, false);
+    } else {
+        constexpr int QP = (J % 4) * 0x55;  // quad_perm [J%4, J%4, J%4, J%4]
+        const int t = __builtin_amdgcn_update_dpp(0, (int)x, QP, 0xF, 0xF, false);
+        if constexpr (G == 4) {
+            return (uint32_t)t;
+        } else {
+            // row_shr:4 into lanes 4-7 and 12-15 of each row, or row_shl:4 into 0-3 and 8-11.
+            constexpr int CTRL = J < 4 ? 0x114 : 0x104;
+            constexpr int BANKS = J < 4 ? 0xA : 0x5;
+            return (uint32_t)__builtin_amdgcn_update_dpp(t, t, CTRL, 0xF, BANKS, false);
+        }
+    }
+}
+
+// Lane J of each group's word x (J < G); gbase is the byte address of lane 0
+// of the caller's group.
+template <int G, int J>
+__device__ __forceinline__ uint32_t group_word(uint32_t x, int gbase) {
+    if constexpr (G <= 16)
+        return group_bcast<G, J>(x);
+    else
+        return (uint32_t)__builtin_amdgcn_ds_bpermute(gbase + J * 4, (int)x);
+}
+
+// f(std::integral_constant<int, I>) for I = B .. E - 1, in order.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 // Walk one chunk of the stream form.  The chunk's entries are loaded once,
 // one 16-byte word per lane per wave-wide block (NP * CHUNK / 64 blocks, read
-// non-temporally: each is used once): lane L of block b of a plane holds
-// position b * EB + L / NG of lane group L % NG, with EB = 64 / NG positions
-// per block.  A lane group takes a word of position p from lane (p % EB) * NG
-// + g with ds_bpermute, so entries cost NP vector-memory instructions per EB
-// positions instead of one or more per position (the vector-memory data path
-// is this kernel's bound).  Each lane group runs through its slice (PMAX
-// positions, padded with dummies) in straight-line code, gathering the N - 1
-// factor rows D positions ahead, so every wait count is exact.  The product
-// is formed in ascending mode order.  Runs of equal rows are summed in
-// registers and added to the LDS tile once per slice.
+// non-temporally: each is used once): lane G * g + j of block b of a plane
+// holds position b * G + j of lane group g.  A lane group takes a word of
+// position p from its own lane p % G (group_word), so entries cost NP
+// vector-memory instructions per G positions and no LDS traffic for groups of
+// up to 16 lanes.  Each lane group runs through its slice (PMAX positions,
+// padded with dummies) in straight-line code, gathering the N - 1 factor rows
+// D positions ahead, so every wait count is exact.  The product is formed in
+// ascending mode order.  Runs of equal rows are summed in registers and added
+// to the LDS tile once per slice.
 // Where a slice's row runs go: the workgroup's LDS tile of the block-row,
 // indexed by the row inside the block-row; lanes past the rank add into tile
 // columns that are never written out.
@@ -595,9 +645,7 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
     constexpr int NP = StreamShape<N>::NP;
     constexpr int NG = 64 / G;
     constexpr int PMAX = CHUNK / NG;
-    constexpr int EB = 64 / NG;
     constexpr int NB = CHUNK / 64;
-    const uint32_t g = (uint32_t)lane / G;
     const uint32_t col = (uint32_t)(lane % G) * V;
     // Uniform bases and 32-bit byte offsets (the host checks that every
     // factor matrix spans < 4 GiB), so each load takes one offset register.
@@ -613,28 +661,31 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
     for (int q = 0; q < NP; ++q)
 #pragma unroll
         for (int b = 0; b < NB; ++b) blk[q][b] = __builtin_nontemporal_load(eb + q * CHUNK + b * 64);
-    const int gsrc = (int)g * 4;
-    auto word = [&](int p, int w) -> uint32_t {
-        return (uint32_t)__builtin_amdgcn_ds_bpermute(gsrc + (p % EB) * NG * 4, (int)blk[w / 4][p / EB][w % 4]);
+    const int gbase = (lane & ~(G - 1)) * 4;  // byte address of lane 0 of this lane's group
+    // Positions are compile-time constants (static_for), so each hand-out
+    // names its source lane in the instruction.
+    auto word = [&](auto P, int w) -> uint32_t {
+        constexpr int p = decltype(P)::value;
+        return group_word<G, p % G>(blk[w / 4][p / G][w % 4], gbase);
     };
     VT raw[D][K];
     float vv[D];
     uint32_t rr[D];
-    auto issue = [&](int p) {
+    auto issue = [&](auto P) {
+        constexpr int p = decltype(P)::value;
 #pragma unroll
         for (int i = 0; i < K; ++i)
-            raw[p % D][i] = *reinterpret_cast<const VT *>(f[i] + (uint64_t)(word(p, i) * rowb + colb));
-        vv[p % D] = __uint_as_float(word(p, K));
-        rr[p % D] = word(p, K + 1);
+            raw[p % D][i] = *reinterpret_cast<const VT *>(f[i] + (uint64_t)(word(P, i) * rowb + colb));
+        vv[p % D] = __uint_as_float(word(P, K));
+        rr[p % D] = word(P, K + 1);
     };
-#pragma unroll
-    for (int p = 0; p < D && p < PMAX; ++p) issue(p);
+    static_for<0, (D < PMAX ? D : PMAX)>(issue);
     float acc[V];
 #pragma unroll
     for (int k = 0; k < V; ++k) acc[k] = 0.f;
     uint32_t cur = Sink::DUMMY;
-#pragma unroll
-    for (int p = 0; p < PMAX; ++p) {
+    static_for<0, PMAX>([&](auto P) {
+        constexpr int p = decltype(P)::value;
         float prod[V], t[V];
         to_float(raw[p % D][0], prod);
         const float v = vv[p % D];
@@ -647,13 +698,13 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
             for (int k = 0; k < V; ++k) prod[k] *= t[k];
         }
         const uint32_t row = rr[p % D];
-        if (p + D < PMAX) issue(p + D);
+        if constexpr (p + D < PMAX) issue(std::integral_constant<int, p + D>{});
         const bool brk = row != cur;
         if (brk && cur != Sink::DUMMY) sink.template add<V>(cur, col, acc);
 #pragma unroll
         for (int k = 0; k < V; ++k) acc[k] = brk ? prod[k] : acc[k] + prod[k];
         cur = row;
-    }
+    });
     if (cur != Sink::DUMMY) sink.template add<V>(cur, col, acc);
 }
 
