@@ -267,6 +267,7 @@ int run_coo(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, con
             h.nunits = u.nunits;
             h.sb_bits = ROW_BITS;
             h.nrows = t->ndims[mode];
+            h.coord24 = coords_fit_u24(t->ndims, N, mode, stride * 4);
             PTI_HIP(launch_stream(N, h, a, sh, false, s), M);
             return 0;
         }

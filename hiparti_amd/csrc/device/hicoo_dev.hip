@@ -698,6 +698,7 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
             h.nunits = c.nunits;
             h.sb_bits = t->sb_bits;
             h.nrows = t->ndims[mode];
+            h.coord24 = coords_fit_u24(t->ndims, N, mode, stride * esz);
             PTI_HIP(launch_stream(N, h, a, sh, bf16, s), M);
         } else {
             PTI_HIP(launch_hicoo(N, hicoo_args(t, mode), a, sh, bf16, s), M);

@@ -37,6 +37,15 @@ inline bool choose_shape(uint32_t R, uint32_t stride, uint32_t out_stride, Shape
     return true;
 }
 
+// StreamArgs::coord24: every non-target mode has at most 2^24 rows and a
+// factor row spans fewer than 2^24 bytes.
+inline uint32_t coords_fit_u24(const uint32_t *ndims, uint32_t N, uint32_t mode, uint64_t row_bytes) {
+    if (row_bytes >= (1ull << 24)) return 0;
+    for (uint32_t m = 0; m < N; ++m)
+        if (m != mode && ndims[m] > (1u << 24)) return 0;
+    return 1;
+}
+
 // A sorted copy's rows cut into 2^sb-row blocks, chunks and units, the
 // structure of a HiCOO mode copy (build_row_units in hicoo_dev.hip).
 struct RowUnits {

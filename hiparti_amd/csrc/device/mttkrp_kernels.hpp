@@ -565,6 +565,10 @@ struct StreamArgs {
     uint32_t nunits;
     uint32_t sb_bits;
     uint32_t nrows;
+    // 1 when every gathered coordinate and the factor row pitch in bytes are
+    // below 2^24: row offsets then take one full-rate 24-bit multiply-add
+    // instead of a quarter-rate 32-bit one.
+    uint32_t coord24;
 };
 
 // Broadcast, inside each group of G lanes, the word that lane J of the group
@@ -578,10 +582,10 @@ template <int G, int J>
 __device__ __forceinline__ uint32_t group_bcast(uint32_t x) {
     static_assert(G == 4 || G == 8 || G == 16, "DPP broadcast covers groups of 4, 8 and 16 lanes");
     if constexpr (G == 16) {
-        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x150 + J, 0xF, 0xF, false);
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x150 + J, 0xF, 0xF, false);
     } else {
         constexpr int QP = (J % 4) * 0x55;  // quad_perm [J%4, J%4, J%4, J%4]
-        const int t = __builtin_amdgcn_update_dpp(0, (int)x, QP, 0xF, 0xF, false);
+        const int t = __builtin_amdgcn_mov_dpp((int)x, QP, 0xF, 0xF, false);
         if constexpr (G == 4) {
             return (uint32_t)t;
         } else {
@@ -601,6 +605,15 @@ __device__ __forceinline__ uint32_t group_word(uint32_t x, int gbase) {
         return group_bcast<G, J>(x);
     else
         return (uint32_t)__builtin_amdgcn_ds_bpermute(gbase + J * 4, (int)x);
+}
+
+// Byte offset of column byte colb in factor row w of pitch rowb bytes.
+template <bool U24>
+__device__ __forceinline__ uint32_t row_offset(uint32_t w, uint32_t rowb, uint32_t colb) {
+    if constexpr (U24)
+        return __umul24(w, rowb) + colb;
+    else
+        return w * rowb + colb;
 }
 
 // f(std::integral_constant<int, I>) for I = B .. E - 1, in order.
@@ -632,12 +645,12 @@ struct TileSink {
     uint32_t tp;
     template <int V>
     __device__ __forceinline__ void add(uint32_t row, uint32_t col, const float *v) const {
-        float *dst = tile + row * tp + col;
+        float *dst = tile + (__umul24(row, tp) + col);  // row < 2^8, tp <= 65
 #pragma unroll
         for (int k = 0; k < V; ++k) atomicAdd(dst + k, v[k]);
     }
 };
-template <int N, int V, int G, int D, typename FT, typename Sink>
+template <int N, int V, int G, int D, typename FT, bool U24, typename Sink>
 __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__restrict__ ent, const Sink &sink,
                                             int lane) {
     using VT = typename Vec<FT, V>::T;
@@ -675,7 +688,7 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
         constexpr int p = decltype(P)::value;
 #pragma unroll
         for (int i = 0; i < K; ++i)
-            raw[p % D][i] = *reinterpret_cast<const VT *>(f[i] + (uint64_t)(word(P, i) * rowb + colb));
+            raw[p % D][i] = *reinterpret_cast<const VT *>(f[i] + (uint64_t)row_offset<U24>(word(P, i), rowb, colb));
         vv[p % D] = __uint_as_float(word(P, K));
         rr[p % D] = word(P, K + 1);
     };
@@ -686,23 +699,35 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
     uint32_t cur = Sink::DUMMY;
     static_for<0, PMAX>([&](auto P) {
         constexpr int p = decltype(P)::value;
+        // prod = v * A_0 * ... * A_{K-2}; the last factor is fused into the
+        // run sum: acc = fma(prod, A_{K-1}, acc), restarted at 0 on a new row.
         float prod[V], t[V];
         to_float(raw[p % D][0], prod);
         const float v = vv[p % D];
 #pragma unroll
         for (int k = 0; k < V; ++k) prod[k] *= v;
 #pragma unroll
-        for (int i = 1; i < K; ++i) {
+        for (int i = 1; i < K - 1; ++i) {
             to_float(raw[p % D][i], t);
 #pragma unroll
             for (int k = 0; k < V; ++k) prod[k] *= t[k];
         }
         const uint32_t row = rr[p % D];
-        if constexpr (p + D < PMAX) issue(std::integral_constant<int, p + D>{});
         const bool brk = row != cur;
         if (brk && cur != Sink::DUMMY) sink.template add<V>(cur, col, acc);
+        if constexpr (K > 1) {
+            to_float(raw[p % D][K - 1], t);
 #pragma unroll
-        for (int k = 0; k < V; ++k) acc[k] = brk ? prod[k] : acc[k] + prod[k];
+            for (int k = 0; k < V; ++k) {
+                float base = brk ? 0.f : acc[k];
+                asm volatile("" : "+v"(base));  // one select and one fma, not fma(.., 0) plus fma plus select
+                acc[k] = __builtin_fmaf(prod[k], t[k], base);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < V; ++k) acc[k] = (brk ? 0.f : acc[k]) + prod[k];
+        }
+        if constexpr (p + D < PMAX) issue(std::integral_constant<int, p + D>{});
         cur = row;
     });
     if (cur != Sink::DUMMY) sink.template add<V>(cur, col, acc);
@@ -742,8 +767,13 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, FT>::OCC))
     __syncthreads();
     const uint32_t c0 = h.unit_chunk[u], c1 = h.unit_chunk[u + 1];
     const uint32_t w0 = __builtin_amdgcn_readfirstlane(wid);
-    for (uint32_t c = c0 + w0; c < c1; c += W)
-        stream_walk<N, V, G, D, FT>(a, h.ent + (uint64_t)c * NP * CHUNK, TileSink{tile, TP}, lane);
+    if (h.coord24) {
+        for (uint32_t c = c0 + w0; c < c1; c += W)
+            stream_walk<N, V, G, D, FT, true>(a, h.ent + (uint64_t)c * NP * CHUNK, TileSink{tile, TP}, lane);
+    } else {
+        for (uint32_t c = c0 + w0; c < c1; c += W)
+            stream_walk<N, V, G, D, FT, false>(a, h.ent + (uint64_t)c * NP * CHUNK, TileSink{tile, TP}, lane);
+    }
     __syncthreads();
     const uint32_t base = h.unit_br[u] << h.sb_bits;
     const bool excl = h.unit_excl[u] != 0;

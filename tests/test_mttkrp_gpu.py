@@ -276,6 +276,37 @@ def test_order3_factors_of_4gib_take_the_tile_kernel():
         del out
 
 
+def test_stream_kernel_with_coordinates_beyond_24_bits():
+    """A non-target mode with more than 2^24 rows sends the stream kernel's
+    row offsets through the 32-bit multiply (the 24-bit multiply-add covers
+    smaller modes): modes 1 and 2 gather rows of mode 0 at and above 2^24,
+    HiCOO and COO, FP32, against the FP64 definition."""
+    import torch
+    from hiparti_amd.device import DevCOO, DevHiCOO
+    ndims = [(1 << 24) + 40, 300, 200]   # mode-0 factor: 2^24 x 16 FP32 = 1 GiB
+    R = 16
+    rng = np.random.default_rng(29)
+    nnz = 30000
+    inds = np.stack([rng.integers(0, d, nnz) for d in ndims]).astype(np.uint32)
+    inds[0, : nnz // 3] = rng.integers((1 << 24) - 8, ndims[0], nnz // 3)
+    keys = np.unique(inds[0].astype(np.uint64) * 60000 + inds[1].astype(np.uint64) * 200 + inds[2], return_index=True)[1]
+    inds = inds[:, keys]
+    t = COOTensor.from_arrays(ndims, inds, rng.random(inds.shape[1]).astype(np.float32))
+    dc = DevCOO(t)
+    dh = DevHiCOO(t.toHiCOO(7, 10, 10))
+    inds, vals = t.inds().astype(np.int64), t.values().astype(np.float64)
+    gen = torch.Generator(device="cuda").manual_seed(6)
+    dfacs = [torch.rand((d, R), generator=gen, device="cuda") for d in ndims]
+    rows = [dfacs[m][torch.from_numpy(inds[m]).cuda()].double().cpu().numpy() for m in range(3)]
+    for mode in (1, 2):
+        a, b = [m for m in range(3) if m != mode]
+        want = np.zeros((ndims[mode], R))
+        np.add.at(want, inds[mode], vals[:, None] * rows[a] * rows[b])
+        for name, d in (("hicoo", dh), ("coo", dc)):
+            got = d.mttkrp(dfacs, mode).cpu().numpy()
+            assert rel_fro(got, want) <= TOL, (name, mode)
+
+
 def test_stream_paths_on_extreme_row_distributions(oracle):
     """The stream kernels' run sums, chunk padding and unit write-out on
     extreme shapes, HiCOO and COO against the FP64 oracle: every nonzero in
