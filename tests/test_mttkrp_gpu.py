@@ -194,6 +194,36 @@ def test_device_api_fp32_and_bf16(oracle):
         assert rel_fro(out.cpu().numpy()[:, :R], f64) < 1e-2
 
 
+@pytest.mark.parametrize("ndims,R", [([3000, 2500, 2000], 16), ([3000, 2500, 2000], 32), ([3000, 2500, 2000], 64),
+                                     ([4000, 3000], 32)])
+def test_stream_kernel_zipf_ranks_and_orders(ndims, R, oracle):
+    """The stream kernel's lane groups of 4, 8 and 16 lanes (ranks 16, 32,
+    64: quad_perm, quad_perm plus row shift, and row_newbcast hand-outs) and
+    order 2, on Zipf indices with block-rows of thousands of nonzeros (long
+    row runs, non-exclusive units).  HiCOO, FP32 and BF16 factors, every
+    mode, against the FP64 oracle."""
+    import torch
+    from hiparti_amd.device import DevHiCOO
+    N = len(ndims)
+    t = COOTensor.synthetic(ndims, 300000, dist=[1] * N, zipf_s=[1.1] * N, permute=[1] * N, seed=11)
+    inds, vals = t.inds(), t.values()
+    rng = np.random.default_rng(12)
+    facs = [rng.random((d, R)).astype(np.float32) for d in ndims]
+    dfacs = [torch.from_numpy(f).cuda() for f in facs]
+    dh = DevHiCOO(t.toHiCOO(7, 14, 14))
+    bf = [f.to(torch.bfloat16) for f in dfacs]
+    rounded = [b.float().cpu().numpy() for b in bf]
+    for mode in range(N):
+        f64 = oracle.mttkrp(inds, vals, facs, mode, kind="f64")
+        out = dh.mttkrp(dfacs, mode)
+        torch.cuda.synchronize()
+        assert rel_fro(out.cpu().numpy()[:, :R], f64) <= TOL, mode
+        if R == 32:
+            out = dh.mttkrp_bf16(bf, mode)
+            torch.cuda.synchronize()
+            assert rel_fro(out.cpu().numpy()[:, :R], oracle.mttkrp(inds, vals, rounded, mode, kind="f64")) <= TOL, mode
+
+
 def test_large_zipf_properties(oracle):
     """Size-independent checks on a c1-shaped tensor (Zipf 1.1, relabelled
     modes, 2^20 x 2^20 x 2^16, sb=7 sk=14) at 4M nonzeros: all-ones factors
