@@ -60,6 +60,8 @@ def parse():
     p.add_argument("--cpu-sample-nnz", type=float, default=8e6)
     p.add_argument("--no-cpd", action="store_true", help="skip the CPD-ALS leg")
     p.add_argument("--no-coo", action="store_true", help="skip the COO (ptiMTTKRP) leg")
+    p.add_argument("--no-completion", action="store_true", help="skip the CP-completion (autograd) leg")
+    p.add_argument("--cmp-nnz", type=float, default=50e6)
     p.add_argument("--cpd-nnz", type=float, default=150e6)
     p.add_argument("--cpd-rank", type=int, default=16)
     p.add_argument("--cpd-iters", type=int, default=20)
@@ -326,6 +328,79 @@ def cpd_leg(args):
                     "fit_delta": abs(runs["fp32"][1] - runs["bf16_fp32"][1])}}
 
 
+def completion_leg(args):
+    """CP-completion loss on the device: order-3 tensor 500k x 500k x 10k with
+    uniform indices, observed values from a planted rank-8 model plus N(0,
+    0.01) noise, rank 32, factors 0.5*U(0,1).  A step is the autograd forward
+    (sampled reconstruction and FP64 loss) plus backward (one MTTKRP of the
+    residuals per mode).  Returns ms per step and per forward, with the
+    forward kernel's bandwidth by the byte convention
+    nnz*(4N + 4 + 4) + sum ndims*R*4 (indices and value read, residual
+    written, factor rows once)."""
+    import torch
+    from hiparti_amd import COOTensor
+    from hiparti_amd.completion import completion_residuals, cp_completion_loss
+    from hiparti_amd.device import DevCOO
+    ndims = [500000, 500000, 10000]
+    nnz = int(args.cmp_nnz)
+    R = 32
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.time()
+    X0 = COOTensor.synthetic(ndims, nnz, seed=args.seed + 7, nthreads=nthreads)
+    inds = X0.inds()
+    X0.free()
+    gen = torch.Generator(device="cuda").manual_seed(args.seed + 7)
+    P = [torch.rand((d, 8), generator=gen, device="cuda", dtype=torch.float64) for d in ndims]
+    planted = torch.ones((nnz, 8), dtype=torch.float64, device="cuda")
+    for m in range(3):
+        planted *= P[m][torch.from_numpy(inds[m].astype("int64")).cuda()]
+    vals = planted.sum(dim=1) + 0.01 * torch.randn(nnz, generator=gen, device="cuda", dtype=torch.float64)
+    del planted, P
+    X = COOTensor.from_arrays(ndims, inds, vals.float().cpu().numpy())
+    del vals, inds
+    dev = DevCOO(X)
+    X.free()
+    params = [(0.5 * torch.rand((d, R), generator=gen, device="cuda")).requires_grad_(True) for d in ndims]
+    torch.cuda.synchronize()
+    log("completion tensor: %d observed entries, ready in %.1f s" % (nnz, time.time() - t0))
+
+    def step():
+        for p in params:
+            p.grad = None
+        loss = cp_completion_loss(dev, params)
+        loss.backward()
+        return loss
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss = step()
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    det = [p.detach() for p in params]
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    for a, b in ev:
+        a.record()
+        completion_residuals(dev, det)
+        b.record()
+    torch.cuda.synchronize()
+    ms_fwd = sum(a.elapsed_time(b) for a, b in ev) / len(ev)
+    fwd_bytes = nnz * (4 * 3 + 4 + 4) + sum(ndims) * R * 4
+    log("completion: %.3f ms per forward+backward, forward %.3f ms, loss %.6g" % (ms_step, ms_fwd, loss.item()))
+    del dev, params
+    torch.cuda.empty_cache()
+    return {"completion": {
+        "workload": "CP-completion loss, order 3, 500k x 500k x 10k, %d observed entries (uniform), planted rank-8 "
+                    "values + N(0, 0.01), R=%d, FP32 factors, FP64 loss" % (nnz, R),
+        "ms_per_forward_backward": round(ms_step, 4),
+        "ms_per_forward": round(ms_fwd, 4),
+        "forward_GBps": round(fwd_bytes / (ms_fwd * 1e-3) / 1e9, 1),
+        "forward_hbm_frac": round(fwd_bytes / (ms_fwd * 1e-3) / 8e12, 4),
+        "loss": loss.item()}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -444,6 +519,8 @@ def main():
         del D, dfacs, outs, bfacs
         torch.cuda.empty_cache()
         extra.update(cpd_leg(args))
+    if not args.no_completion:
+        extra.update(completion_leg(args))
     if not args.no_cpu:
         t0 = time.time()
         cpu_ms, kind, z_s, k_s = cpu_baseline(*cpu_args, args.cpu_sample_nnz, nthreads)

@@ -157,6 +157,24 @@ def test_values_override_with_own_values_matches_plain_mttkrp(oracle):
         assert rel_fro(a, ref) <= TOL, n
 
 
+def test_replaced_and_own_values_alternate(oracle):
+    """Replaced-value MTTKRPs (the segmented-scan COO kernel reading the
+    values through the sort permutation) alternate with own-value ones (the
+    stream kernel on the same tensor): random, then doubled own values, on
+    every mode, each against the FP64 oracle."""
+    coo, dev, inds, vals, facs, tfacs = setup((300, 2000, 1500), 40000, 32, seed=13)
+    rng = np.random.default_rng(14)
+    other = rng.standard_normal(len(vals)).astype(np.float32)
+    for n in range(3):
+        ref_own = oracle.mttkrp(inds, vals, facs, n, kind="f64")
+        got = mttkrp_values(dev, torch.from_numpy(other).cuda(), tfacs, n).cpu().numpy()
+        assert rel_fro(got, oracle.mttkrp(inds, other, facs, n, kind="f64")) <= TOL, n
+        assert rel_fro(dev.mttkrp(tfacs, n).cpu().numpy(), ref_own) <= TOL, n
+        got = mttkrp_values(dev, torch.from_numpy(2 * vals).cuda(), tfacs, n).cpu().numpy()
+        assert rel_fro(got, 2 * ref_own) <= TOL, n
+        assert rel_fro(dev.mttkrp(tfacs, n).cpu().numpy(), ref_own) <= TOL, n
+
+
 def test_partial_gradients_and_descent():
     coo, dev, inds, vals, facs, tfacs = setup((50, 40, 30), 6000, 8, seed=21, scale=0.5)
     params = [t.clone().requires_grad_(i != 1) for i, t in enumerate(tfacs)]
