@@ -158,6 +158,7 @@ SIGNATURES = {
     "ptiDevMTTKRPCOO": (C.c_int, [_vp, _P(_vp), ptiIndex, ptiIndex, _P(ptiIndex), ptiIndex, _vp]),
     "ptiDevMTTKRPCOOValues": (C.c_int, [_vp, _vp, _P(_vp), ptiIndex, ptiIndex, _P(ptiIndex), ptiIndex, _vp]),
     "ptiDevCOOCompletionForward": (C.c_int, [_vp, _P(_vp), ptiIndex, ptiIndex, _vp, _vp, _vp]),
+    "ptiDevCOOCompletionGrad": (C.c_int, [_vp, _vp, _P(_vp), ptiIndex, ptiIndex, ptiIndex, _vp]),
     "ptiDevMTTKRPHiCOO_BF16": (C.c_int, [_vp, _P(_vp), _vp, ptiIndex, ptiIndex, ptiIndex,
                                          _P(ptiIndex), ptiIndex, _vp]),
     "ptiDevGram": (C.c_int, [_vp, ptiIndex, ptiIndex, ptiIndex, _vp, _vp]),

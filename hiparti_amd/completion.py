@@ -13,9 +13,13 @@ HIP kernels:
             writes the residual e_z = x_z - y_z and the fixed-order FP64 sum
             of e_z^2 / 2.
   backward  dL/dA_n = -MTTKRP(E, n), where E is X's sparsity pattern carrying
-            the residuals as values: ptiDevMTTKRPCOOValues, the same COO
-            MTTKRP kernel as ptiMTTKRP (reference src/sptensor/mttkrp.c:58-91)
-            with the values replaced.
+            the residuals as values (reference ptiMTTKRP,
+            src/sptensor/mttkrp.c:58-91, on that tensor):
+            ptiDevCOOCompletionGrad, which for ranks <= 64 on orders <= 5
+            recomputes each residual inside the MTTKRP's walk (the output
+            row's own factor row staged in LDS) instead of reading the saved
+            residuals through the sort permutation, and otherwise runs
+            ptiDevMTTKRPCOOValues on the saved residuals.
 
 The gradient is exact for the loss above (the sampled reconstruction is
 multilinear in the factors), so gradient checks against FP64 finite
@@ -74,9 +78,8 @@ class CPCompletionLoss(torch.autograd.Function):
                 continue
             out = torch.empty((facs[n].shape[0], R), dtype=torch.float32, device=facs[n].device)
             ptrs = (_vp * (N + 1))(*[f.data_ptr() for f in facs], out.data_ptr())
-            check(lib().ptiDevMTTKRPCOOValues(dev.handle, _vp(resid.data_ptr()), ptrs, R, R,
-                                              _order(n, N, None), n, _stream_ptr()),
-                  "ptiDevMTTKRPCOOValues")
+            check(lib().ptiDevCOOCompletionGrad(dev.handle, _vp(resid.data_ptr()), ptrs, R, R, n, _stream_ptr()),
+                  "ptiDevCOOCompletionGrad")
             grads.append(out.mul_(scale))
         return tuple(grads)
 

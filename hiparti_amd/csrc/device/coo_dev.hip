@@ -290,6 +290,56 @@ int run_coo(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, con
     return 0;
 }
 
+// CP-completion gradient of `mode`: out = MTTKRP of the residuals
+// e_z = x_z - sum_r prod_m A_m[i_m][r] (dmats[0..N-1] = A_m, dmats[N] = out).
+// With the stream kernel's shapes (ranks <= 64, orders <= 5, every gathered
+// factor matrix < 4 GiB) the residuals are formed inside the walk from A_mode's
+// rows; otherwise this is the MTTKRP of `resid` (the forward's residuals in
+// storage order) on the segmented-scan kernel.
+int completion_grad(ptiDevCOO *t, const float *resid, float *const dmats[], ptiIndex R, ptiIndex stride,
+                    ptiIndex mode, hipStream_t s, const char *M) {
+    const uint32_t N = t->nmodes;
+    if (mode >= N) PTI_FAIL(PTIERR_VALUE_ERROR, M, "mode out of range");
+    if (R == 0 || stride < R) PTI_FAIL(PTIERR_SHAPE_MISMATCH, M, "bad rank or stride");
+    Shape sh;
+    bool ok = R <= 64 && choose_shape(R, stride, stride, sh) && stream_supported(N, sh.G);
+    for (uint32_t m = 0; m < N && ok; ++m)
+        if (m != mode && (uint64_t)t->ndims[m] * stride * 4 >= (1ull << 32)) ok = false;
+    if (ok) {
+        PTI_CHECK(prepare_mode(t, mode, s), M);
+        float *out = dmats[N];
+        PTI_HIP(zero_floats(out, (uint64_t)t->ndims[mode] * stride, s), M);
+        if (t->nnz == 0) return 0;
+        PTI_CHECK(ensure_coo_stream(t, mode, 64u / sh.G, s, M), M);
+        FactorArgs a;
+        for (uint32_t m = 0; m < N; ++m) a.fac[m] = dmats[m];
+        a.out = out;
+        a.R = R;
+        a.stride = stride;
+        a.out_stride = stride;
+        a.mode = mode;
+        a.order[0] = mode;
+        for (uint32_t i = 1; i < N; ++i) a.order[i] = i - 1 < mode ? i - 1 : i;
+        const RowUnits &u = t->units[mode];
+        StreamArgs h{};
+        h.ent = t->stream[mode].as<uint4>();
+        h.unit_chunk = u.unit_chunk.as<uint32_t>();
+        h.unit_br = u.unit_br.as<uint32_t>();
+        h.unit_excl = u.unit_excl.as<uint32_t>();
+        h.nunits = u.nunits;
+        h.sb_bits = ROW_BITS;
+        h.nrows = t->ndims[mode];
+        h.coord24 = coords_fit_u24(t->ndims, N, mode, stride * 4);
+        PTI_HIP(launch_completion_grad(N, h, a, sh, s), M);
+        return 0;
+    }
+    if (!resid && t->nnz) PTI_FAIL(PTIERR_VALUE_ERROR, M, "residuals must not be NULL for this shape");
+    ptiIndex order[8];
+    order[0] = mode;
+    for (uint32_t i = 1; i < N; ++i) order[i] = i - 1 < mode ? i - 1 : i;
+    return run_coo(t, dmats, R, stride, order, mode, s, M, resid);
+}
+
 void coo_view(const ptiDevCOO *t, uint32_t *nmodes, const ptiIndex **ndims, uint64_t *nnz, const float **vals) {
     *nmodes = t->nmodes;
     *ndims = t->ndims;
@@ -325,6 +375,12 @@ int ptiDevMTTKRPCOOValues(ptiDevCOO *t, float const *dvals, float *const dmats[]
     if (!dvals && t->nnz) PTI_FAIL(PTIERR_VALUE_ERROR, "DevCOO MTTKRP values", "values must not be NULL");
     return run_coo(t, dmats, R, stride, mats_order, mode, static_cast<hipStream_t>(stream), "DevCOO MTTKRP values",
                    dvals);
+}
+
+int ptiDevCOOCompletionGrad(ptiDevCOO *t, float const *dresid, float *const dmats[], ptiIndex R, ptiIndex stride,
+                            ptiIndex mode, void *stream) {
+    return completion_grad(t, dresid, dmats, R, stride, mode, static_cast<hipStream_t>(stream),
+                           "DevCOO Completion gradient");
 }
 
 int ptiDevCOOCompletionForward(ptiDevCOO *t, float *const dfacs[], ptiIndex R, ptiIndex stride, float *dresid,

@@ -616,6 +616,20 @@ __device__ __forceinline__ uint32_t row_offset(uint32_t w, uint32_t rowb, uint32
         return w * rowb + colb;
 }
 
+// Sum of x over each group of G (4, 8 or 16) lanes, in every lane of the
+// group, with DPP moves: swaps inside pairs and quads (quad_perm), then the
+// mirrored 8-lane half-row (row_half_mirror) and the mirrored 16-lane row
+// (row_mirror), each of which pairs every lane with one of the other half.
+template <int G>
+__device__ __forceinline__ float group_sum(float x) {
+    static_assert(G == 4 || G == 8 || G == 16, "DPP sums cover groups of 4, 8 and 16 lanes");
+    x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    if constexpr (G >= 8) x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x141, 0xF, 0xF, false));
+    if constexpr (G >= 16) x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x140, 0xF, 0xF, false));
+    return x;
+}
+
 // f(std::integral_constant<int, I>) for I = B .. E - 1, in order.
 template <int B, int E, typename F>
 __device__ __forceinline__ void static_for(F &&f) {
@@ -650,9 +664,13 @@ struct TileSink {
         for (int k = 0; k < V; ++k) atomicAdd(dst + k, v[k]);
     }
 };
-template <int N, int V, int G, int D, typename FT, bool U24, typename Sink>
+// RESID (CP-completion gradient): each entry's value x is replaced by its
+// residual e = x - sum_r prod_m A_m[i_m][r], the target row's factor row
+// read from trow (the block-row's rows of A_mode in LDS, pitch TRP floats,
+// rows past the block-row clamped to trow_last); dummies contribute 0.
+template <int N, int V, int G, int D, typename FT, bool U24, bool RESID = false, typename Sink>
 __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__restrict__ ent, const Sink &sink,
-                                            int lane) {
+                                            int lane, const float *trow = nullptr, uint32_t trow_last = 0) {
     using VT = typename Vec<FT, V>::T;
     constexpr int K = N - 1;
     constexpr int NP = StreamShape<N>::NP;
@@ -699,6 +717,35 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
     uint32_t cur = Sink::DUMMY;
     static_for<0, PMAX>([&](auto P) {
         constexpr int p = decltype(P)::value;
+        if constexpr (RESID) {
+            constexpr uint32_t TRP = G * V + 4;
+            float pp[V], t[V];
+            to_float(raw[p % D][0], pp);
+#pragma unroll
+            for (int i = 1; i < K; ++i) {
+                to_float(raw[p % D][i], t);
+#pragma unroll
+                for (int k = 0; k < V; ++k) pp[k] *= t[k];
+            }
+            const uint32_t row = rr[p % D];
+            const float *tr = trow + (row < trow_last ? row : trow_last) * TRP + col;
+            float part = 0.f;
+#pragma unroll
+            for (int k = 0; k < V; ++k) part = __builtin_fmaf(pp[k], tr[k], part);
+            part = group_sum<G>(part);
+            const float e = row == Sink::DUMMY ? 0.f : vv[p % D] - part;
+            const bool brk = row != cur;
+            if (brk && cur != Sink::DUMMY) sink.template add<V>(cur, col, acc);
+#pragma unroll
+            for (int k = 0; k < V; ++k) {
+                float base = brk ? 0.f : acc[k];
+                asm volatile("" : "+v"(base));
+                acc[k] = __builtin_fmaf(e, pp[k], base);
+            }
+            if constexpr (p + D < PMAX) issue(std::integral_constant<int, p + D>{});
+            cur = row;
+            return;
+        }
         // prod = v * A_0 * ... * A_{K-2}; the last factor is fused into the
         // run sum: acc = fma(prod, A_{K-1}, acc), restarted at 0 on a new row.
         float prod[V], t[V];
@@ -776,6 +823,60 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, FT>::OCC))
     }
     __syncthreads();
     const uint32_t base = h.unit_br[u] << h.sb_bits;
+    const bool excl = h.unit_excl[u] != 0;
+    for (uint32_t i = tid; i < rows * GV; i += NT) {
+        const uint32_t r = i / GV, c = i % GV;
+        const uint32_t orow = base + r;
+        if (c >= a.R || orow >= h.nrows) continue;
+        const float v = tile[r * TP + c];
+        if (v == 0.f) continue;
+        float *dst = a.out + (uint64_t)orow * a.out_stride + c;
+        if (excl)
+            *dst = v;
+        else
+            unsafeAtomicAdd(dst, v);
+    }
+}
+
+// CP-completion gradient on a COO stream form: out[i_n] += e_z * prod_{m != n}
+// A_m[i_m] with the residual e_z = x_z - sum_r prod_m A_m[i_m][r] formed in
+// the walk, so the residuals need not be gathered through the sort
+// permutation.  As hicoo_stream_kernel (FP32 factors), plus the unit's
+// block-row of A_n (2^sb rows, zero past the tensor's rows and the rank) in
+// LDS after the tile.
+template <int N, int V, int G, int W>
+__global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, float>::OCC))
+    completion_grad_kernel(StreamArgs h, FactorArgs a) {
+    constexpr int D = StreamConfig<N, V, float>::D;
+    constexpr int NP = StreamShape<N>::NP;
+    extern __shared__ float tile[];  // (1 << sb_bits) rows x TP floats, then as many rows x TRP
+    constexpr int GV = G * V;
+    constexpr int TP = GV + 1;
+    constexpr int TRP = GV + 4;
+    constexpr int NT = W * 64;
+    const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+    const uint32_t u = remap_block(blockIdx.x, gridDim.x);   // gridDim.x == nunits
+    const uint32_t rows = 1u << h.sb_bits;
+    const uint32_t base = h.unit_br[u] << h.sb_bits;
+    float *trow = tile + rows * TP;
+    const float *An = static_cast<const float *>(a.fac[a.mode]);
+    for (uint32_t i = tid; i < rows * TP; i += NT) tile[i] = 0.f;
+    for (uint32_t i = tid; i < rows * GV; i += NT) {
+        const uint32_t r = i / GV, c = i % GV;
+        trow[r * TRP + c] = (c < a.R && base + r < h.nrows) ? An[(uint64_t)(base + r) * a.stride + c] : 0.f;
+    }
+    __syncthreads();
+    const uint32_t c0 = h.unit_chunk[u], c1 = h.unit_chunk[u + 1];
+    const uint32_t w0 = __builtin_amdgcn_readfirstlane(wid);
+    const TileSink sink{tile, TP};
+    if (h.coord24) {
+        for (uint32_t c = c0 + w0; c < c1; c += W)
+            stream_walk<N, V, G, D, float, true, true>(a, h.ent + (uint64_t)c * NP * CHUNK, sink, lane, trow, rows - 1);
+    } else {
+        for (uint32_t c = c0 + w0; c < c1; c += W)
+            stream_walk<N, V, G, D, float, false, true>(a, h.ent + (uint64_t)c * NP * CHUNK, sink, lane, trow, rows - 1);
+    }
+    __syncthreads();
     const bool excl = h.unit_excl[u] != 0;
     for (uint32_t i = tid; i < rows * GV; i += NT) {
         const uint32_t r = i / GV, c = i % GV;

@@ -76,6 +76,15 @@ int ptiDevMTTKRPCOOValues(ptiDevCOO *t, float const *dvals, float * const dmats[
 int ptiDevCOOCompletionForward(ptiDevCOO *t, float * const dfacs[], ptiIndex R, ptiIndex stride,
                                float *dresid, double *dloss, void *stream);
 
+/* CP-completion gradient (no counterpart in the reference; the backward of
+ * the loss above): dmats[nmodes] = MTTKRP of the residuals
+ *   e_z = x_z - sum_r prod_m A_m[i_m][r],   A_m = dmats[m] (m < nmodes),
+ * i.e. -dL/dA_mode.  Ranks <= 64 on orders <= 5 form e_z inside the MTTKRP
+ * from the factors and ignore dresid; other shapes take the MTTKRP of dresid
+ * (the forward's residuals, storage order).  Output row stride `stride`. */
+int ptiDevCOOCompletionGrad(ptiDevCOO *t, float const *dresid, float * const dmats[], ptiIndex R,
+                            ptiIndex stride, ptiIndex mode, void *stream);
+
 /* FP64 dense step of CP-ALS on device data.  The reference does this work on
  * the host with BLAS/LAPACK in FP32 (src/matrix/matrix_solver.c:32-83,
  * src/matrix/matrix.c:340-572, src/ktensor/ktensor.c:77-240).  Matrices are
