@@ -332,11 +332,12 @@ def completion_leg(args):
     """CP-completion loss on the device: order-3 tensor 500k x 500k x 10k with
     uniform indices, observed values from a planted rank-8 model plus N(0,
     0.01) noise, rank 32, factors 0.5*U(0,1).  A step is the autograd forward
-    (sampled reconstruction and FP64 loss) plus backward (one MTTKRP of the
-    residuals per mode).  Returns ms per step and per forward, with the
-    forward kernel's bandwidth by the byte convention
-    nnz*(4N + 4 + 4) + sum ndims*R*4 (indices and value read, residual
-    written, factor rows once)."""
+    plus backward of the loss with every factor requiring a gradient: the
+    loss and the three gradient MTTKRPs of the residuals (residuals formed
+    inside each MTTKRP), then their scaling.  Also times the forward-only
+    kernel (sampled reconstruction, residuals and FP64 loss), with its
+    bandwidth by the byte convention nnz*(4N + 4 + 4) + sum ndims*R*4
+    (indices and value read, residual written, factor rows once)."""
     import torch
     from hiparti_amd import COOTensor
     from hiparti_amd.completion import completion_residuals, cp_completion_loss

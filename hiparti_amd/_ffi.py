@@ -159,6 +159,8 @@ SIGNATURES = {
     "ptiDevMTTKRPCOOValues": (C.c_int, [_vp, _vp, _P(_vp), ptiIndex, ptiIndex, _P(ptiIndex), ptiIndex, _vp]),
     "ptiDevCOOCompletionForward": (C.c_int, [_vp, _P(_vp), ptiIndex, ptiIndex, _vp, _vp, _vp]),
     "ptiDevCOOCompletionGrad": (C.c_int, [_vp, _vp, _P(_vp), ptiIndex, ptiIndex, ptiIndex, _vp]),
+    "ptiDevCOOCompletionFused": (C.c_int, [_vp, ptiIndex, ptiIndex]),
+    "ptiDevCOOCompletionLossGrad": (C.c_int, [_vp, _P(_vp), _P(_vp), ptiIndex, ptiIndex, _vp, _vp]),
     "ptiDevMTTKRPHiCOO_BF16": (C.c_int, [_vp, _P(_vp), _vp, ptiIndex, ptiIndex, ptiIndex,
                                          _P(ptiIndex), ptiIndex, _vp]),
     "ptiDevGram": (C.c_int, [_vp, ptiIndex, ptiIndex, ptiIndex, _vp, _vp]),

@@ -40,8 +40,14 @@ class CPCompletionLoss(torch.autograd.Function):
 
     dev_coo: a DevCOO holding the observed entries.  A_m: contiguous float32
     CUDA tensors of shape (ndims[m], R), 1 <= R <= 256.  Returns a 0-d
-    float64 tensor.  The residuals of the last forward are kept on the
-    context for backward."""
+    float64 tensor.
+
+    When every factor needs a gradient and the shape allows it
+    (ptiDevCOOCompletionFused: ranks <= 64 on orders <= 5), the forward runs
+    ptiDevCOOCompletionLossGrad, which returns the loss and all N gradient
+    MTTKRPs from N fused passes, and the backward only scales them by the
+    incoming gradient.  Otherwise the forward keeps the residuals and the
+    backward runs one ptiDevCOOCompletionGrad per needed mode."""
 
     @staticmethod
     def forward(ctx, dev, *facs):
@@ -54,34 +60,52 @@ class CPCompletionLoss(torch.autograd.Function):
             if f.shape[1] != R or f.shape[0] < dev.ndims[m]:
                 raise ValueError("factor %d must have shape (>= %d, %d)" % (m, dev.ndims[m], R))
         device = facs[0].device
-        resid = torch.empty(dev.nnz, dtype=torch.float32, device=device)
         loss = torch.empty((), dtype=torch.float64, device=device)
         ptrs = (_vp * N)(*[f.data_ptr() for f in facs])
+        ctx.dev = dev
+        ctx.grads = None
+        if all(ctx.needs_input_grad[1:]) and lib().ptiDevCOOCompletionFused(dev.handle, R, R) == 1:
+            grads = [_grad_buffer(f, dev.ndims[m]) for m, f in enumerate(facs)]
+            gptrs = (_vp * N)(*[g.data_ptr() for g in grads])
+            check(lib().ptiDevCOOCompletionLossGrad(dev.handle, ptrs, gptrs, R, R, _vp(loss.data_ptr()),
+                                                    _stream_ptr()), "ptiDevCOOCompletionLossGrad")
+            ctx.grads = grads
+            return loss
+        resid = torch.empty(dev.nnz, dtype=torch.float32, device=device)
         check(lib().ptiDevCOOCompletionForward(dev.handle, ptrs, R, R, _vp(resid.data_ptr()),
                                                _vp(loss.data_ptr()), _stream_ptr()),
               "ptiDevCOOCompletionForward")
-        ctx.dev = dev
         ctx.save_for_backward(resid, *facs)
         return loss
 
     @staticmethod
     def backward(ctx, gloss):
+        scale = -gloss.to(torch.float32)
+        if ctx.grads is not None:
+            return (None,) + tuple(g * scale for g in ctx.grads)
         dev = ctx.dev
         resid, *facs = ctx.saved_tensors
         N = dev.nmodes
         R = facs[0].shape[1]
-        scale = -gloss.to(torch.float32)
         grads = [None]
         for n in range(N):
             if not ctx.needs_input_grad[n + 1]:
                 grads.append(None)
                 continue
-            out = torch.empty((facs[n].shape[0], R), dtype=torch.float32, device=facs[n].device)
+            out = _grad_buffer(facs[n], dev.ndims[n])
             ptrs = (_vp * (N + 1))(*[f.data_ptr() for f in facs], out.data_ptr())
             check(lib().ptiDevCOOCompletionGrad(dev.handle, _vp(resid.data_ptr()), ptrs, R, R, n, _stream_ptr()),
                   "ptiDevCOOCompletionGrad")
             grads.append(out.mul_(scale))
         return tuple(grads)
+
+
+def _grad_buffer(f, nrows):
+    """Gradient storage shaped like factor f; the kernels write its first
+    nrows rows, rows past the tensor's extent are zero."""
+    if f.shape[0] > nrows:
+        return torch.zeros(f.shape, dtype=torch.float32, device=f.device)
+    return torch.empty(f.shape, dtype=torch.float32, device=f.device)
 
 
 def cp_completion_loss(dev, facs):

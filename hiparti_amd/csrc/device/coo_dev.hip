@@ -290,22 +290,30 @@ int run_coo(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, con
     return 0;
 }
 
+// Whether the CP-completion gradient of `mode` runs completion_grad_kernel:
+// ranks <= 64 with the stream kernel's shapes (orders <= 5, lane groups of
+// <= 16 lanes) and every gathered factor matrix < 4 GiB.
+bool grad_fused(const ptiDevCOO *t, ptiIndex R, ptiIndex stride, ptiIndex mode, Shape &sh) {
+    const uint32_t N = t->nmodes;
+    bool ok = R >= 1 && R <= 64 && stride >= R && choose_shape(R, stride, stride, sh) && stream_supported(N, sh.G);
+    for (uint32_t m = 0; m < N && ok; ++m)
+        if (m != mode && (uint64_t)t->ndims[m] * stride * 4 >= (1ull << 32)) ok = false;
+    return ok;
+}
+
 // CP-completion gradient of `mode`: out = MTTKRP of the residuals
 // e_z = x_z - sum_r prod_m A_m[i_m][r] (dmats[0..N-1] = A_m, dmats[N] = out).
-// With the stream kernel's shapes (ranks <= 64, orders <= 5, every gathered
-// factor matrix < 4 GiB) the residuals are formed inside the walk from A_mode's
-// rows; otherwise this is the MTTKRP of `resid` (the forward's residuals in
-// storage order) on the segmented-scan kernel.
+// On completion_grad_kernel (grad_fused) the residuals are formed inside the
+// walk from A_mode's rows, and loss_part, when given, receives per unit the
+// sum of e_z^2; otherwise this is the MTTKRP of `resid` (the forward's
+// residuals in storage order) on the segmented-scan kernel.
 int completion_grad(ptiDevCOO *t, const float *resid, float *const dmats[], ptiIndex R, ptiIndex stride,
-                    ptiIndex mode, hipStream_t s, const char *M) {
+                    ptiIndex mode, hipStream_t s, const char *M, double *loss_part = nullptr) {
     const uint32_t N = t->nmodes;
     if (mode >= N) PTI_FAIL(PTIERR_VALUE_ERROR, M, "mode out of range");
     if (R == 0 || stride < R) PTI_FAIL(PTIERR_SHAPE_MISMATCH, M, "bad rank or stride");
     Shape sh;
-    bool ok = R <= 64 && choose_shape(R, stride, stride, sh) && stream_supported(N, sh.G);
-    for (uint32_t m = 0; m < N && ok; ++m)
-        if (m != mode && (uint64_t)t->ndims[m] * stride * 4 >= (1ull << 32)) ok = false;
-    if (ok) {
+    if (grad_fused(t, R, stride, mode, sh)) {
         PTI_CHECK(prepare_mode(t, mode, s), M);
         float *out = dmats[N];
         PTI_HIP(zero_floats(out, (uint64_t)t->ndims[mode] * stride, s), M);
@@ -330,14 +338,51 @@ int completion_grad(ptiDevCOO *t, const float *resid, float *const dmats[], ptiI
         h.sb_bits = ROW_BITS;
         h.nrows = t->ndims[mode];
         h.coord24 = coords_fit_u24(t->ndims, N, mode, stride * 4);
-        PTI_HIP(launch_completion_grad(N, h, a, sh, s), M);
+        PTI_HIP(launch_completion_grad(N, h, a, sh, loss_part, s), M);
         return 0;
     }
+    if (loss_part) PTI_FAIL(PTIERR_VALUE_ERROR, M, "no fused gradient for this shape");
     if (!resid && t->nnz) PTI_FAIL(PTIERR_VALUE_ERROR, M, "residuals must not be NULL for this shape");
     ptiIndex order[8];
     order[0] = mode;
     for (uint32_t i = 1; i < N; ++i) order[i] = i - 1 < mode ? i - 1 : i;
     return run_coo(t, dmats, R, stride, order, mode, s, M, resid);
+}
+
+// Loss and every mode's gradient MTTKRP in N passes of
+// completion_grad_kernel; mode 0's pass also sums e_z^2 per unit, reduced in
+// a fixed order.  Every mode must take the fused kernel (grad_fused).
+int completion_loss_grad(ptiDevCOO *t, float *const dfacs[], float *const dgrads[], ptiIndex R, ptiIndex stride,
+                         double *dloss, hipStream_t s, const char *M) {
+    const uint32_t N = t->nmodes;
+    Shape sh;
+    for (uint32_t n = 0; n < N; ++n)
+        if (!grad_fused(t, R, stride, n, sh)) PTI_FAIL(PTIERR_VALUE_ERROR, M, "no fused gradient for this shape");
+    if (t->nnz == 0) {
+        for (uint32_t n = 0; n < N; ++n) PTI_HIP(zero_floats(dgrads[n], (uint64_t)t->ndims[n] * stride, s), M);
+        PTI_HIP(zero_floats(reinterpret_cast<float *>(dloss), 2, s), M);
+        return 0;
+    }
+    for (uint32_t n = 0; n < N; ++n) {
+        float *dm[9];
+        for (uint32_t m = 0; m < N; ++m) dm[m] = dfacs[m];
+        dm[N] = dgrads[n];
+        double *part = nullptr;
+        if (n == 0) {
+            PTI_CHECK(prepare_mode(t, 0, s), M);
+            PTI_CHECK(ensure_coo_stream(t, 0, 64u / sh.G, s, M), M);
+            const uint32_t nu = t->units[0].nunits;
+            if (t->partials.bytes < (size_t)std::max<uint32_t>(nu, 1) * sizeof(double))
+                PTI_HIP(t->partials.alloc((size_t)std::max<uint32_t>(nu, 1) * sizeof(double)), M);
+            part = t->partials.as<double>();
+        }
+        PTI_CHECK(completion_grad(t, nullptr, dm, R, stride, n, s, M, part), M);
+        if (n == 0) {
+            hipLaunchKernelGGL(half_sum_kernel, dim3(1), dim3(256), 0, s, part, t->units[0].nunits, dloss);
+            PTI_HIP(hipGetLastError(), M);
+        }
+    }
+    return 0;
 }
 
 void coo_view(const ptiDevCOO *t, uint32_t *nmodes, const ptiIndex **ndims, uint64_t *nnz, const float **vals) {
@@ -375,6 +420,19 @@ int ptiDevMTTKRPCOOValues(ptiDevCOO *t, float const *dvals, float *const dmats[]
     if (!dvals && t->nnz) PTI_FAIL(PTIERR_VALUE_ERROR, "DevCOO MTTKRP values", "values must not be NULL");
     return run_coo(t, dmats, R, stride, mats_order, mode, static_cast<hipStream_t>(stream), "DevCOO MTTKRP values",
                    dvals);
+}
+
+int ptiDevCOOCompletionFused(ptiDevCOO const *t, ptiIndex R, ptiIndex stride) {
+    Shape sh;
+    for (uint32_t n = 0; n < t->nmodes; ++n)
+        if (!grad_fused(t, R, stride, n, sh)) return 0;
+    return 1;
+}
+
+int ptiDevCOOCompletionLossGrad(ptiDevCOO *t, float *const dfacs[], float *const dgrads[], ptiIndex R,
+                                ptiIndex stride, double *dloss, void *stream) {
+    return completion_loss_grad(t, dfacs, dgrads, R, stride, dloss, static_cast<hipStream_t>(stream),
+                                "DevCOO Completion loss and gradient");
 }
 
 int ptiDevCOOCompletionGrad(ptiDevCOO *t, float const *dresid, float *const dmats[], ptiIndex R, ptiIndex stride,

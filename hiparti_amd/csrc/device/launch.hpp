@@ -65,7 +65,8 @@ constexpr bool stream_supported(uint32_t N, int G) { return N <= 5 && G <= 16; }
     hipError_t launch_coo_##N(const COOArgs &c, const FactorArgs &a, Shape sh, hipStream_t s);        \
     hipError_t launch_completion_##N(const COOArgs &c, const FactorArgs &a, Shape sh, float *resid,    \
                                      double *partials, hipStream_t s);                                \
-    hipError_t launch_completion_grad_##N(const StreamArgs &h, const FactorArgs &a, Shape sh, hipStream_t s);
+    hipError_t launch_completion_grad_##N(const StreamArgs &h, const FactorArgs &a, Shape sh, double *loss_part, \
+                                          hipStream_t s);
 
 PTI_DECLARE_LAUNCHERS(2)
 PTI_DECLARE_LAUNCHERS(3)
@@ -107,8 +108,8 @@ inline hipError_t launch_coo(uint32_t N, const COOArgs &c, const FactorArgs &a, 
 }
 
 inline hipError_t launch_completion_grad(uint32_t N, const StreamArgs &h, const FactorArgs &a, Shape sh,
-                                         hipStream_t s) {
-#define PTI_CALL(n) launch_completion_grad_##n(h, a, sh, s)
+                                         double *loss_part, hipStream_t s) {
+#define PTI_CALL(n) launch_completion_grad_##n(h, a, sh, loss_part, s)
     PTI_DISPATCH_N(N, PTI_CALL)
 #undef PTI_CALL
 }

@@ -74,24 +74,24 @@ hipError_t stream_vg(const StreamArgs &h, const FactorArgs &a, bool bf16, hipStr
 }
 
 template <int V, int G>
-hipError_t grad_vg(const StreamArgs &h, const FactorArgs &a, hipStream_t s) {
+hipError_t grad_vg(const StreamArgs &h, const FactorArgs &a, double *loss_part, hipStream_t s) {
     if constexpr (G * V > 64 || !stream_supported(N, G)) {
         return hipErrorInvalidValue;
     } else {
         if (h.nunits == 0) return hipSuccess;
         const size_t rows = (size_t)1 << h.sb_bits;
         const size_t lds = rows * (G * V + 1) * sizeof(float) + rows * (G * V + 4) * sizeof(float);
-        hipLaunchKernelGGL((completion_grad_kernel<N, V, G, 8>), dim3(h.nunits), dim3(512), lds, s, h, a);
+        hipLaunchKernelGGL((completion_grad_kernel<N, V, G, 8>), dim3(h.nunits), dim3(512), lds, s, h, a, loss_part);
         return hipGetLastError();
     }
 }
 
 template <int V>
-hipError_t grad_v(const StreamArgs &h, const FactorArgs &a, int G, hipStream_t s) {
+hipError_t grad_v(const StreamArgs &h, const FactorArgs &a, int G, double *loss_part, hipStream_t s) {
     switch (G) {
-        case 4: return grad_vg<V, 4>(h, a, s);
-        case 8: return grad_vg<V, 8>(h, a, s);
-        case 16: return grad_vg<V, 16>(h, a, s);
+        case 4: return grad_vg<V, 4>(h, a, loss_part, s);
+        case 8: return grad_vg<V, 8>(h, a, loss_part, s);
+        case 16: return grad_vg<V, 16>(h, a, loss_part, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -183,11 +183,11 @@ hipError_t PTI_CAT(launch_stream_, PTI_NMODES)(const StreamArgs &h, const Factor
 }
 
 hipError_t PTI_CAT(launch_completion_grad_, PTI_NMODES)(const StreamArgs &h, const FactorArgs &a, Shape sh,
-                                                        hipStream_t s) {
+                                                        double *loss_part, hipStream_t s) {
     switch (sh.V) {
-        case 4: return grad_v<4>(h, a, sh.G, s);
-        case 2: return grad_v<2>(h, a, sh.G, s);
-        case 1: return grad_v<1>(h, a, sh.G, s);
+        case 4: return grad_v<4>(h, a, sh.G, loss_part, s);
+        case 2: return grad_v<2>(h, a, sh.G, loss_part, s);
+        case 1: return grad_v<1>(h, a, sh.G, loss_part, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -670,7 +670,8 @@ struct TileSink {
 // rows past the block-row clamped to trow_last); dummies contribute 0.
 template <int N, int V, int G, int D, typename FT, bool U24, bool RESID = false, typename Sink>
 __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__restrict__ ent, const Sink &sink,
-                                            int lane, const float *trow = nullptr, uint32_t trow_last = 0) {
+                                            int lane, const float *trow = nullptr, uint32_t trow_last = 0,
+                                            double *esq = nullptr) {
     using VT = typename Vec<FT, V>::T;
     constexpr int K = N - 1;
     constexpr int NP = StreamShape<N>::NP;
@@ -734,6 +735,7 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
             for (int k = 0; k < V; ++k) part = __builtin_fmaf(pp[k], tr[k], part);
             part = group_sum<G>(part);
             const float e = row == Sink::DUMMY ? 0.f : vv[p % D] - part;
+            if (esq && col == 0) *esq += (double)e * (double)e;   // one lane per entry
             const bool brk = row != cur;
             if (brk && cur != Sink::DUMMY) sink.template add<V>(cur, col, acc);
 #pragma unroll
@@ -844,9 +846,11 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, FT>::OCC))
 // permutation.  As hicoo_stream_kernel (FP32 factors), plus the unit's
 // block-row of A_n (2^sb rows, zero past the tensor's rows and the rank) in
 // LDS after the tile.
+// loss_part (optional): per unit, the sum of e_z^2 over its entries in a
+// fixed order (each wave's chunks in turn, then the waves in order).
 template <int N, int V, int G, int W>
 __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, float>::OCC))
-    completion_grad_kernel(StreamArgs h, FactorArgs a) {
+    completion_grad_kernel(StreamArgs h, FactorArgs a, double *loss_part) {
     constexpr int D = StreamConfig<N, V, float>::D;
     constexpr int NP = StreamShape<N>::NP;
     extern __shared__ float tile[];  // (1 << sb_bits) rows x TP floats, then as many rows x TRP
@@ -869,12 +873,28 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, float>::OCC))
     const uint32_t c0 = h.unit_chunk[u], c1 = h.unit_chunk[u + 1];
     const uint32_t w0 = __builtin_amdgcn_readfirstlane(wid);
     const TileSink sink{tile, TP};
+    double esq = 0.0;
+    double *pe = loss_part ? &esq : nullptr;
     if (h.coord24) {
         for (uint32_t c = c0 + w0; c < c1; c += W)
-            stream_walk<N, V, G, D, float, true, true>(a, h.ent + (uint64_t)c * NP * CHUNK, sink, lane, trow, rows - 1);
+            stream_walk<N, V, G, D, float, true, true>(a, h.ent + (uint64_t)c * NP * CHUNK, sink, lane, trow, rows - 1,
+                                                       pe);
     } else {
         for (uint32_t c = c0 + w0; c < c1; c += W)
-            stream_walk<N, V, G, D, float, false, true>(a, h.ent + (uint64_t)c * NP * CHUNK, sink, lane, trow, rows - 1);
+            stream_walk<N, V, G, D, float, false, true>(a, h.ent + (uint64_t)c * NP * CHUNK, sink, lane, trow,
+                                                        rows - 1, pe);
+    }
+    if (loss_part) {
+        __shared__ double wsum[W];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) esq += __shfl_xor(esq, off, 64);
+        if (lane == 0) wsum[wid] = esq;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0.0;
+            for (int w = 0; w < W; ++w) t += wsum[w];
+            loss_part[u] = t;
+        }
     }
     __syncthreads();
     const bool excl = h.unit_excl[u] != 0;

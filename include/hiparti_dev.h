@@ -85,6 +85,17 @@ int ptiDevCOOCompletionForward(ptiDevCOO *t, float * const dfacs[], ptiIndex R, 
 int ptiDevCOOCompletionGrad(ptiDevCOO *t, float const *dresid, float * const dmats[], ptiIndex R,
                             ptiIndex stride, ptiIndex mode, void *stream);
 
+/* CP-completion loss and every mode's gradient MTTKRP in one call (no
+ * counterpart in the reference; for autograd callers that need both):
+ * dgrads[n] = MTTKRP of the residuals for mode n (row stride `stride`) and
+ * *dloss = 0.5 * sum_z e_z^2 (device double, fixed summation order), with
+ * the residuals formed inside the MTTKRPs, so no forward pass is needed.
+ * Only for shapes where ptiDevCOOCompletionFused returns 1 (every mode runs
+ * the fused kernel); otherwise PTIERR_VALUE_ERROR and nothing is written. */
+int ptiDevCOOCompletionFused(ptiDevCOO const *t, ptiIndex R, ptiIndex stride);
+int ptiDevCOOCompletionLossGrad(ptiDevCOO *t, float * const dfacs[], float * const dgrads[], ptiIndex R,
+                                ptiIndex stride, double *dloss, void *stream);
+
 /* FP64 dense step of CP-ALS on device data.  The reference does this work on
  * the host with BLAS/LAPACK in FP32 (src/matrix/matrix_solver.c:32-83,
  * src/matrix/matrix.c:340-572, src/ktensor/ktensor.c:77-240).  Matrices are
