@@ -175,6 +175,29 @@ def test_replaced_and_own_values_alternate(oracle):
         assert rel_fro(dev.mttkrp(tfacs, n).cpu().numpy(), ref_own) <= TOL, n
 
 
+@pytest.mark.parametrize("ndims,nnz,R", [((200, 150, 100), 70000, 64), ((40, 50, 60), 3000, 16),
+                                         ((30, 20, 25, 15), 2000, 10), ((9, 8, 7, 6, 5), 4000, 33)])
+def test_fused_and_residual_paths_agree(ndims, nnz, R):
+    """With every factor requiring a gradient the loss and gradients come
+    from the fused passes (ranks <= 64 on orders <= 5; 8-, 4- and 16-lane
+    groups here), with one factor frozen from the forward kernel plus one
+    gradient MTTKRP per mode; both against the FP64 loss, and the gradients
+    of the shared factors against each other.  Order 5 at rank 33 has no
+    fused form and takes the residual path both times."""
+    coo, dev, inds, vals, facs, tfacs = setup(ndims, nnz, R, seed=17, scale=0.6)
+    ref = np_loss(inds, vals, facs)
+    full = [t.clone().requires_grad_(True) for t in tfacs]
+    loss_f = cp_completion_loss(dev, full)
+    loss_f.backward()
+    part = [t.clone().requires_grad_(i != 0) for i, t in enumerate(tfacs)]
+    loss_p = cp_completion_loss(dev, part)
+    loss_p.backward()
+    assert abs(loss_f.item() - ref) <= TOL * ref
+    assert abs(loss_p.item() - ref) <= TOL * ref
+    for n in range(1, len(ndims)):
+        assert rel_fro(full[n].grad.cpu().numpy(), part[n].grad.cpu().numpy()) <= TOL, n
+
+
 def test_partial_gradients_and_descent():
     coo, dev, inds, vals, facs, tfacs = setup((50, 40, 30), 6000, 8, seed=21, scale=0.5)
     params = [t.clone().requires_grad_(i != 1) for i, t in enumerate(tfacs)]
