@@ -29,9 +29,10 @@
 //
 // Stream kernel (hicoo_stream_kernel): the same units and LDS tile, but each
 // chunk is read already decoded from HBM (see StreamShape) and the nonzeros
-// of a block-row are in output-row order across its chunks.  It runs orders
-// 2-5 with lane groups of at most 16 lanes and factor matrices below 4 GiB;
-// the tile kernel takes the rest.
+// of a block-row are in output-row order across its chunks; lane groups take
+// their entries from their own lanes with DPP moves.  It runs orders 2-5
+// with lane groups of at most 16 lanes and factor matrices below 4 GiB; the
+// tile kernel takes the rest.
 //
 // COO kernel (coo_mttkrp_kernel).  Per target mode the device keeps a copy
 // sorted by the output index; each run of equal rows in a chunk is added to
@@ -39,7 +40,13 @@
 // allow, the sorted copy is instead cut into 2^7-row blocks, chunks and units
 // like a HiCOO mode copy and run by hicoo_stream_kernel (LDS tile per block);
 // coo_mttkrp_kernel serves the rest and the replaced-values form
-// (CP-completion residuals).
+// (CP-completion residuals passed in).
+//
+// CP-completion (completion_forward_kernel, completion_grad_kernel).  The
+// forward gathers every factor row of each observed entry in storage order
+// and writes its residual; the gradient kernel is the stream kernel on a COO
+// copy's stream form that forms each residual itself, from the output row's
+// factor row staged in LDS, and accumulates residual * product.
 //
 // Run reduction (COO kernel).  Within a step the NG contributions are
 // combined by a segmented inclusive scan over groups keyed on the output row;
@@ -785,9 +792,10 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
 // Pipeline of the stream walk: gather depth D (about 32 registers of rows in
 // flight) and the minimum resident waves per SIMD the register budget is
 // sized for.  Measured on the 100M-nonzero order-3 Zipf tensor at R = 32
-// (FP32): D = 4 at five waves per SIMD (111 VGPRs) was the fastest of D =
-// 2..8 at two to eight waves; deeper rings at fewer waves were up to 10%
-// slower.
+// (FP32): with the DPP entry hand-out (80 VGPRs at D = 4) D = 4, 5, 6 and 8
+// at four to six waves per SIMD were within 1% of each other, the kernel
+// being limited by the vector-memory path's throughput rather than by
+// latency; D = 4 keeps the smallest register budget.
 template <int N, int V, typename FT>
 struct StreamConfig {
     static constexpr int ROW_REGS = (N - 1) * V * (int)sizeof(FT) / 4;
