@@ -30,6 +30,7 @@ struct ptiDevCOO {
     pti::dev::Buffer partials;                // completion forward: per-workgroup sums
     pti::dev::RowUnits units[8];              // [target mode] row blocks / chunks / units of the sorted copy
     pti::dev::Buffer stream[8];               // [target mode] stream form of the sorted copy
+    pti::dev::Buffer unit_order[8];           // [target mode] its units by decreasing chunk count
     uint32_t stream_ng[8] = {0};              // its lane-group count (0: not built)
     bool ready[8] = {false};
     std::mutex mu;
@@ -217,6 +218,7 @@ int ensure_coo_stream(ptiDevCOO *t, ptiIndex mode, uint32_t ng, hipStream_t s, c
                        t->svals[mode].as<float>(), u.chunk_z.as<uint32_t>(), u.nchunks, N, (uint32_t)mode, ng,
                        t->stream[mode].as<uint4>());
     PTI_HIP(hipGetLastError(), M);
+    PTI_CHECK(build_unit_order(u.unit_chunk.as<uint32_t>(), u.nunits, t->unit_order[mode], s, M), M);
     t->stream_ng[mode] = ng;
     return 0;
 }
@@ -259,7 +261,7 @@ int run_coo(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, con
             a.mode = mode;
             for (uint32_t i = 0; i < N; ++i) a.order[i] = mats_order[i];
             const RowUnits &u = t->units[mode];
-            StreamArgs h;
+            StreamArgs h{};
             h.ent = t->stream[mode].as<uint4>();
             h.unit_chunk = u.unit_chunk.as<uint32_t>();
             h.unit_br = u.unit_br.as<uint32_t>();
@@ -268,6 +270,7 @@ int run_coo(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, con
             h.sb_bits = ROW_BITS;
             h.nrows = t->ndims[mode];
             h.coord24 = coords_fit_u24(t->ndims, N, mode, stride * 4);
+            h.unit_order = t->unit_order[mode].as<uint32_t>();
             PTI_HIP(launch_stream(N, h, a, sh, false, s), M);
             return 0;
         }
@@ -338,6 +341,7 @@ int completion_grad(ptiDevCOO *t, const float *resid, float *const dmats[], ptiI
         h.sb_bits = ROW_BITS;
         h.nrows = t->ndims[mode];
         h.coord24 = coords_fit_u24(t->ndims, N, mode, stride * 4);
+        h.unit_order = t->unit_order[mode].as<uint32_t>();
         PTI_HIP(launch_completion_grad(N, h, a, sh, loss_part, s), M);
         return 0;
     }

@@ -55,7 +55,7 @@ struct ptiDevHiCOOMode {
     // Stream form of the copy (16 bytes per chunk position and plane), built
     // on first use for the lane-group count `stream_ng` of the rank shape.
     mutable std::mutex stream_mu;
-    mutable pti::dev::Buffer stream;
+    mutable pti::dev::Buffer stream, unit_order;   // unit_order: units by decreasing chunk count
     mutable uint32_t stream_ng = 0;
 };
 
@@ -365,6 +365,12 @@ __global__ void __launch_bounds__(256) stream_fill_kernel(HiCOOArgs h, uint32_t 
     }
 }
 
+__global__ void __launch_bounds__(256) unit_size_kernel(const uint32_t *__restrict__ unit_chunk, uint32_t n,
+                                                        uint32_t *__restrict__ sz) {
+    for (uint32_t u = blockIdx.x * 256 + threadIdx.x; u < n; u += gridDim.x * 256)
+        sz[u] = unit_chunk[u + 1] - unit_chunk[u];
+}
+
 // Row blocks of nonzeros sorted by output row: flag[z] = 1 where row >> sb
 // changes.
 __global__ void __launch_bounds__(256) row_block_flag_kernel(const uint32_t *__restrict__ rows, uint64_t n, uint32_t sb,
@@ -633,6 +639,7 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
     hipLaunchKernelGGL(stream_fill_kernel, grid, dim3(256), 0, s, hicoo_args(t, mode), c.nchunks, N, mode, ng,
                        spos.as<uint32_t>(), blk_of.as<uint32_t>(), c.stream.as<uint4>());
     PTI_HIP(hipGetLastError(), M);
+    PTI_CHECK(build_unit_order(c.unit_chunk.as<uint32_t>(), c.nunits, c.unit_order, s, M), M);
     // The temporaries are released on return: the queued work must finish.
     PTI_HIP(hipStreamSynchronize(s), M);
     c.stream_ng = ng;
@@ -690,7 +697,7 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
         if (fits && stream_supported(N, sh.G)) {
             PTI_CHECK(ensure_stream(t, mode, 64u / sh.G, s, M), M);
             const ptiDevHiCOOMode &c = t->mc[mode];
-            StreamArgs h;
+            StreamArgs h{};
             h.ent = c.stream.as<uint4>();
             h.unit_chunk = c.unit_chunk.as<uint32_t>();
             h.unit_br = c.unit_br.as<uint32_t>();
@@ -699,6 +706,7 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
             h.sb_bits = t->sb_bits;
             h.nrows = t->ndims[mode];
             h.coord24 = coords_fit_u24(t->ndims, N, mode, stride * esz);
+            h.unit_order = c.unit_order.as<uint32_t>();
             PTI_HIP(launch_stream(N, h, a, sh, bf16, s), M);
         } else {
             PTI_HIP(launch_hicoo(N, hicoo_args(t, mode), a, sh, bf16, s), M);
@@ -716,6 +724,31 @@ void hicoo_view(const ptiDevHiCOO *t, uint32_t *nmodes, const ptiIndex **ndims, 
     *vals = nullptr;
     for (uint32_t m = 0; m < t->nmodes && !*vals; ++m)
         if (t->mc[m].ready) *vals = t->mc[m].vals.as<float>();
+}
+
+// order = the units' indices by decreasing chunk count (stable), for
+// StreamArgs::unit_order.
+int build_unit_order(const uint32_t *unit_chunk, uint32_t nunits, Buffer &order, hipStream_t s, const char *M) {
+    PTI_HIP(order.alloc((size_t)nunits * 4), M);
+    if (nunits == 0) return 0;
+    Buffer sz, ssz, ids, tmp;
+    PTI_HIP(sz.alloc((size_t)nunits * 4), M);
+    PTI_HIP(ssz.alloc((size_t)nunits * 4), M);
+    PTI_HIP(ids.alloc((size_t)nunits * 4), M);
+    hipLaunchKernelGGL(unit_size_kernel, grid_n(nunits), dim3(256), 0, s, unit_chunk, nunits, sz.as<uint32_t>());
+    hipLaunchKernelGGL(iota_u32_kernel, grid_n(nunits), dim3(256), 0, s, ids.as<uint32_t>(), (uint64_t)nunits);
+    PTI_HIP(hipGetLastError(), M);
+    size_t tb = 0;
+    PTI_HIP(rocprim::radix_sort_pairs_desc(nullptr, tb, sz.as<uint32_t>(), ssz.as<uint32_t>(), ids.as<uint32_t>(),
+                                           order.as<uint32_t>(), (size_t)nunits, 0, 8, s),
+            M);
+    PTI_HIP(tmp.alloc(tb), M);
+    PTI_HIP(rocprim::radix_sort_pairs_desc(tmp.ptr, tb, sz.as<uint32_t>(), ssz.as<uint32_t>(), ids.as<uint32_t>(),
+                                           order.as<uint32_t>(), (size_t)nunits, 0, 8, s),
+            M);
+    // The temporaries are released on return.
+    PTI_HIP(hipStreamSynchronize(s), M);
+    return 0;
 }
 
 // Cuts n nonzeros sorted by output row (rows[]) into blocks of 2^sb rows,

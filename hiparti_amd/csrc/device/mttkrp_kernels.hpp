@@ -576,7 +576,15 @@ struct StreamArgs {
     // below 2^24: row offsets then take one full-rate 24-bit multiply-add
     // instead of a quarter-rate 32-bit one.
     uint32_t coord24;
+    // Optional: units in order of decreasing chunk count; workgroup b runs
+    // unit_order[b], so the hardware's round-robin dealing gives every XCD
+    // an even share and the small units run last.  Null: remap_block order.
+    const uint32_t *unit_order;
 };
+
+__device__ __forceinline__ uint32_t stream_unit(const StreamArgs &h) {
+    return h.unit_order ? h.unit_order[blockIdx.x] : remap_block(blockIdx.x, gridDim.x);
+}
 
 // Broadcast, inside each group of G lanes, the word that lane J of the group
 // holds.  Groups of 4, 8 and 16 lanes use DPP moves (VALU work: the LDS pipe,
@@ -818,7 +826,7 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, FT>::OCC))
     constexpr int TP = GV + 1;
     constexpr int NT = W * 64;
     const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
-    const uint32_t u = remap_block(blockIdx.x, gridDim.x);   // gridDim.x == nunits
+    const uint32_t u = stream_unit(h);   // gridDim.x == nunits
     const uint32_t rows = 1u << h.sb_bits;
     for (uint32_t i = tid; i < rows * TP; i += NT) tile[i] = 0.f;
     __syncthreads();
@@ -867,7 +875,7 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, float>::OCC))
     constexpr int TRP = GV + 4;
     constexpr int NT = W * 64;
     const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
-    const uint32_t u = remap_block(blockIdx.x, gridDim.x);   // gridDim.x == nunits
+    const uint32_t u = stream_unit(h);   // gridDim.x == nunits
     const uint32_t rows = 1u << h.sb_bits;
     const uint32_t base = h.unit_br[u] << h.sb_bits;
     float *trow = tile + rows * TP;
