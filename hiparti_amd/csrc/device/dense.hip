@@ -79,19 +79,34 @@ __global__ void __launch_bounds__(256) gram_partial_kernel(const float *__restri
     for (int k = 0; k < 4; ++k) out[ty * GT + 4 * tx + k] = acc[k];
 }
 
+// Sums the nsplit partial tiles: workgroup (tile, y) takes the tile's 64
+// entries y*64 .. y*64+63 (two tile rows), entries outside R x R skipped; its
+// 4 waves sum contiguous quarters of the partials, in order, then the
+// quarters are added in order, so the result does not depend on timing.
 __global__ void __launch_bounds__(256) gram_reduce_kernel(const double *__restrict__ partials, uint32_t nsplit,
                                                           uint32_t R, double *__restrict__ G) {
+    __shared__ double red[4][64];
     uint32_t tp, tq;
     tile_of(blockIdx.x, cdiv(R, GT), tp, tq);
-    for (int e = threadIdx.x; e < GT * GT; e += 256) {
-        const uint32_t p = tp * GT + e / GT, q = tq * GT + e % GT;
-        double s = 0.0;
-        for (uint32_t k = 0; k < nsplit; ++k) s += partials[((uint64_t)blockIdx.x * nsplit + k) * (GT * GT) + e];
+    const int tid = threadIdx.x, quarter = tid >> 6, lane = tid & 63;
+    const uint32_t e = blockIdx.y * 64 + lane;
+    const uint32_t p = tp * GT + e / GT, q = tq * GT + e % GT;
+    if (tp * GT + (blockIdx.y * 64) / GT >= R) return;   // both tile rows past the rank: whole workgroup
+    const uint32_t per = (nsplit + 3) / 4;
+    const uint32_t k0 = quarter * per, k1 = k0 + per < nsplit ? k0 + per : nsplit;
+    double sum = 0.0;
+    if (p < R && q < R) {
+        const double *src = partials + (uint64_t)blockIdx.x * nsplit * (GT * GT) + e;
+#pragma unroll 8
+        for (uint32_t k = k0; k < k1; ++k) sum += src[(uint64_t)k * (GT * GT)];
+    }
+    red[quarter][lane] = sum;
+    __syncthreads();
+    if (quarter == 0 && p < R && q < R && (tp < tq || p <= q)) {
         // Diagonal tiles hold both (p, q) and (q, p); take the upper one.
-        if (p < R && q < R && (tp < tq || p <= q)) {
-            G[(size_t)p * R + q] = s;
-            G[(size_t)q * R + p] = s;
-        }
+        const double t = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+        G[(size_t)p * R + q] = t;
+        G[(size_t)q * R + p] = t;
     }
 }
 
@@ -271,21 +286,25 @@ __global__ void __launch_bounds__(256) apply_kernel(const float *__restrict__ Mr
     }
 }
 
-// Per-workgroup column partials over a row range; thread (phase, lane):
-// column lane + 64 g, rows i0 + phase, step 4.
+// Per-workgroup column partials over a row range: the 256 threads are CW
+// column lanes (CW = 16, 32 or 64, the smallest covering R, columns done CW at
+// a time) times 256 / CW row phases; phase f takes rows i0 + f, i0 + f + P,
+// ...; the phases are combined in order.
 __global__ void __launch_bounds__(256) colstat_partial_kernel(const float *__restrict__ A,
                                                               const float *__restrict__ B, uint64_t nrows,
                                                               uint32_t R, uint32_t stride, int kind,
                                                               uint64_t rows_per_split, double *__restrict__ partials) {
-    __shared__ double red[4][64];
-    const int tid = threadIdx.x, phase = tid >> 6, lane = tid & 63;
+    __shared__ double red[256];
+    const uint32_t CW = R > 32 ? 64u : (R > 16 ? 32u : 16u);
+    const uint32_t P = 256 / CW;
+    const uint32_t tid = threadIdx.x, phase = tid / CW, lane = tid % CW;
     const uint64_t i0 = (uint64_t)blockIdx.x * rows_per_split;
     const uint64_t i1 = i0 + rows_per_split < nrows ? i0 + rows_per_split : nrows;
-    for (uint32_t g = 0; g < R; g += 64) {
+    for (uint32_t g = 0; g < R; g += CW) {
         const uint32_t c = g + lane;
         double acc = 0.0;
         if (c < R) {
-            for (uint64_t i = i0 + phase; i < i1; i += 4) {
+            for (uint64_t i = i0 + phase; i < i1; i += P) {
                 const double a = A[i * stride + c];
                 if (kind == COL_2NORM)
                     acc = fma(a, a, acc);
@@ -295,26 +314,44 @@ __global__ void __launch_bounds__(256) colstat_partial_kernel(const float *__res
                     acc = fma(a, (double)B[i * stride + c], acc);
             }
         }
-        red[phase][lane] = acc;
+        red[tid] = acc;
         __syncthreads();
         if (phase == 0 && c < R) {
-            double t = red[0][lane];
-            for (int p = 1; p < 4; ++p) t = kind == COL_MAXNORM ? (red[p][lane] > t ? red[p][lane] : t) : t + red[p][lane];
+            double t = red[lane];
+            for (uint32_t f = 1; f < P; ++f) {
+                const double v = red[f * CW + lane];
+                t = kind == COL_MAXNORM ? (v > t ? v : t) : t + v;
+            }
             partials[(uint64_t)blockIdx.x * R + c] = t;
         }
         __syncthreads();
     }
 }
 
+// Combines the nsplit column partials: workgroup y takes columns
+// y*64 .. y*64+63; its 4 waves reduce contiguous quarters of the partials in
+// order, then the quarters are combined in order.
 __global__ void __launch_bounds__(256) colstat_reduce_kernel(const double *__restrict__ partials, uint32_t nsplit,
                                                              uint32_t R, int kind, double *__restrict__ out,
                                                              double *__restrict__ mul) {
-    for (uint32_t c = threadIdx.x; c < R; c += 256) {
-        double t = 0.0;
-        for (uint32_t k = 0; k < nsplit; ++k) {
+    __shared__ double red[4][64];
+    const int tid = threadIdx.x, quarter = tid >> 6, lane = tid & 63;
+    const uint32_t c = blockIdx.x * 64 + lane;
+    const uint32_t per = (nsplit + 3) / 4;
+    const uint32_t k0 = quarter * per, k1 = k0 + per < nsplit ? k0 + per : nsplit;
+    double t = 0.0;
+    if (c < R) {
+#pragma unroll 8
+        for (uint32_t k = k0; k < k1; ++k) {
             const double v = partials[(uint64_t)k * R + c];
             t = kind == COL_MAXNORM ? (v > t ? v : t) : t + v;
         }
+    }
+    red[quarter][lane] = t;
+    __syncthreads();
+    if (quarter == 0 && c < R) {
+        t = red[0][lane];
+        for (int f = 1; f < 4; ++f) t = kind == COL_MAXNORM ? (red[f][lane] > t ? red[f][lane] : t) : t + red[f][lane];
         if (kind == COL_2NORM) t = sqrt(t);
         if (kind == COL_MAXNORM && t < 1.0) t = 1.0;
         out[c] = t;
@@ -441,7 +478,7 @@ int dense_gram(const float *A, uint64_t nrows, uint32_t R, uint32_t stride, doub
     PTI_CHECK(w.ensure(R, M), M);
     if (stride < R) PTI_FAIL(PTIERR_SHAPE_MISMATCH, M, "stride < rank");
     const uint32_t ntiles = gram_tiles(R);
-    uint32_t nsplit = cdiv(nrows, 2048);
+    uint32_t nsplit = cdiv(nrows, 1024);
     const uint32_t cap = MAX_PARTS / ntiles;
     if (nsplit > cap) nsplit = cap;
     if (nsplit == 0) nsplit = 1;
@@ -450,7 +487,8 @@ int dense_gram(const float *A, uint64_t nrows, uint32_t R, uint32_t stride, doub
     hipLaunchKernelGGL(gram_partial_kernel, dim3(nsplit, ntiles), dim3(256), 0, s, A, nrows, R, stride, per, nsplit,
                        w.partials.as<double>());
     PTI_HIP(hipGetLastError(), M);
-    hipLaunchKernelGGL(gram_reduce_kernel, dim3(ntiles), dim3(256), 0, s, w.partials.as<double>(), nsplit, R, G);
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3(ntiles, GT * GT / 64), dim3(256), 0, s, w.partials.as<double>(),
+                       nsplit, R, G);
     PTI_HIP(hipGetLastError(), M);
     return 0;
 }
@@ -481,8 +519,8 @@ int dense_colstat(float *A, const float *B, uint64_t nrows, uint32_t R, uint32_t
     hipLaunchKernelGGL(colstat_partial_kernel, dim3(nsplit), dim3(256), 0, s, A, B, nrows, R, stride, (int)kind, per,
                        w.partials.as<double>());
     PTI_HIP(hipGetLastError(), M);
-    hipLaunchKernelGGL(colstat_reduce_kernel, dim3(1), dim3(256), 0, s, w.partials.as<double>(), nsplit, R, (int)kind,
-                       out, mul);
+    hipLaunchKernelGGL(colstat_reduce_kernel, dim3(cdiv(R, 64)), dim3(256), 0, s, w.partials.as<double>(), nsplit, R,
+                       (int)kind, out, mul);
     PTI_HIP(hipGetLastError(), M);
     if (scale && nrows) {
         hipLaunchKernelGGL(scale_kernel, grid_elems(nrows * R), dim3(256), 0, s, A, nrows, R, stride, out);
