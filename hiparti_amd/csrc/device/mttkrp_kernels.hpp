@@ -686,7 +686,10 @@ struct TileSink {
 template <int N, int V, int G, int D, typename FT, bool U24, bool RESID = false, typename Sink>
 __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__restrict__ ent, const Sink &sink,
                                             int lane, const float *trow = nullptr, uint32_t trow_last = 0,
-                                            double *esq = nullptr) {
+                                            double *esq_sum = nullptr) {
+    // Sum of squared residuals (RESID with esq_sum): kept in a register and
+    // added to *esq_sum once at the end.
+    double esq = 0.0;
     using VT = typename Vec<FT, V>::T;
     constexpr int K = N - 1;
     constexpr int NP = StreamShape<N>::NP;
@@ -750,7 +753,7 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
             for (int k = 0; k < V; ++k) part = __builtin_fmaf(pp[k], tr[k], part);
             part = group_sum<G>(part);
             const float e = row == Sink::DUMMY ? 0.f : vv[p % D] - part;
-            if (esq && col == 0) *esq += (double)e * (double)e;   // one lane per entry
+            if (col == 0) esq += (double)e * (double)e;   // one lane per entry
             const bool brk = row != cur;
             if (brk && cur != Sink::DUMMY) sink.template add<V>(cur, col, acc);
 #pragma unroll
@@ -795,6 +798,8 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
         cur = row;
     });
     if (cur != Sink::DUMMY) sink.template add<V>(cur, col, acc);
+    if constexpr (RESID)
+        if (esq_sum) *esq_sum += esq;
 }
 
 // Pipeline of the stream walk: gather depth D (about 32 registers of rows in
@@ -864,8 +869,10 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, FT>::OCC))
 // LDS after the tile.
 // loss_part (optional): per unit, the sum of e_z^2 over its entries in a
 // fixed order (each wave's chunks in turn, then the waves in order).
+// Four resident waves per SIMD: the residual's group sum and the FP64 loss
+// term need registers beyond the plain walk's budget at 16-lane groups.
 template <int N, int V, int G, int W>
-__global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, float>::OCC))
+__global__ void __launch_bounds__(W * 64, 4)
     completion_grad_kernel(StreamArgs h, FactorArgs a, double *loss_part) {
     constexpr int D = StreamConfig<N, V, float>::D;
     constexpr int NP = StreamShape<N>::NP;
@@ -889,8 +896,8 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, float>::OCC))
     const uint32_t c0 = h.unit_chunk[u], c1 = h.unit_chunk[u + 1];
     const uint32_t w0 = __builtin_amdgcn_readfirstlane(wid);
     const TileSink sink{tile, TP};
-    double esq = 0.0;
-    double *pe = loss_part ? &esq : nullptr;
+    double esq = 0.0;   // this lane's sum of squared residuals (used when loss_part is set)
+    double *pe = &esq;
     if (h.coord24) {
         for (uint32_t c = c0 + w0; c < c1; c += W)
             stream_walk<N, V, G, D, float, true, true>(a, h.ent + (uint64_t)c * NP * CHUNK, sink, lane, trow, rows - 1,
