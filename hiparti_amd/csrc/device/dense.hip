@@ -57,20 +57,23 @@ __global__ void __launch_bounds__(256) gram_partial_kernel(const float *__restri
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     const uint64_t i0 = (uint64_t)blockIdx.x * rows_per_split;
     const uint64_t i1 = i0 + rows_per_split < nrows ? i0 + rows_per_split : nrows;
+    // A diagonal tile (tp == tq, every tile for R <= 32) stages its rows once.
+    const bool diag = tp == tq;
+    float(*qs)[GT + 1] = diag ? sp : sq;
     for (uint64_t base = i0; base < i1; base += GROWS) {
         for (int e = tid; e < GROWS * GT; e += 256) {
             const int r = e / GT, c = e % GT;
             const uint64_t i = base + r;
             const uint32_t cp = tp * GT + c, cq = tq * GT + c;
             sp[r][c] = (i < i1 && cp < R) ? A[i * stride + cp] : 0.f;
-            sq[r][c] = (i < i1 && cq < R) ? A[i * stride + cq] : 0.f;
+            if (!diag) sq[r][c] = (i < i1 && cq < R) ? A[i * stride + cq] : 0.f;
         }
         __syncthreads();
 #pragma unroll 4
         for (int r = 0; r < GROWS; ++r) {
             const double a = sp[r][ty];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) acc[k] = fma(a, (double)sq[r][4 * tx + k], acc[k]);
+            for (int k = 0; k < 4; ++k) acc[k] = fma(a, (double)qs[r][4 * tx + k], acc[k]);
         }
         __syncthreads();
     }
