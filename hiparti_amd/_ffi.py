@@ -105,6 +105,9 @@ SIGNATURES = {
     "ptiFreeSparseTensor": (None, [_P(ptiSparseTensor)]),
     "ptiLoadSparseTensor": (C.c_int, [_P(ptiSparseTensor), ptiIndex, C.c_char_p]),
     "ptiDumpSparseTensor": (C.c_int, [_P(ptiSparseTensor), ptiIndex, _vp]),
+    "ptiDumpSparseTensorHiCOO": (C.c_int, [_P(ptiSparseTensorHiCOO), _vp]),
+    "ptiDumpMatrix": (C.c_int, [_P(ptiMatrix), _vp]),
+    "ptiDumpKruskalTensor": (C.c_int, [_P(ptiKruskalTensor), _vp]),
     "ptiSparseTensorSortIndex": (None, [_P(ptiSparseTensor), C.c_int, C.c_int]),
     "SparseTensorFrobeniusNormSquared": (C.c_double, [_P(ptiSparseTensor)]),
     "ptiResizeIndexVector": (C.c_int, [_P(ptiIndexVector), ptiNnzIndex]),

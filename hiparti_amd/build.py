@@ -71,7 +71,9 @@ def build(verbose=False, jobs=None):
                 print(out)
     objs = [c[c.index("-o") + 1] for c in cmds]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-pthread"])
+    # Only the C ABI is exported (exports.map); the C++ internals stay local.
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-pthread", "-Wl,--version-script=" + os.path.join(CSRC, "exports.map")])
     os.replace(tmp, LIB)
     return LIB
 

@@ -30,7 +30,7 @@ import ctypes as C
 import torch
 
 from ._lib import check, lib
-from .device import _check_factor, _order, _stream_ptr
+from .device import _check_factor, _check_factors, _check_mode, _check_out, _order, _stream_ptr
 
 _vp = C.c_void_p
 
@@ -118,9 +118,7 @@ def completion_residuals(dev, facs):
     """Forward only: (loss, residuals) with residuals in the tensor's storage
     order.  No autograd graph is recorded."""
     N = dev.nmodes
-    R = facs[0].shape[1]
-    for m, f in enumerate(facs):
-        _check_factor(f, torch.float32, "factor %d" % m)
+    R = _check_factors(facs, dev.ndims, torch.float32, None)
     resid = torch.empty(dev.nnz, dtype=torch.float32, device=facs[0].device)
     loss = torch.empty((), dtype=torch.float64, device=facs[0].device)
     ptrs = (_vp * N)(*[f.data_ptr() for f in facs])
@@ -133,14 +131,14 @@ def mttkrp_values(dev, values, facs, mode, out=None):
     """MTTKRP of X's sparsity pattern carrying `values` (nnz float32, storage
     order) instead of X's own values."""
     N = dev.nmodes
-    R = facs[0].shape[1]
-    for m, f in enumerate(facs):
-        _check_factor(f, torch.float32, "factor %d" % m)
+    _check_mode(mode, N)
+    R = _check_factors(facs, dev.ndims, torch.float32, None, skip=mode)
     if not (values.is_cuda and values.dtype == torch.float32 and values.is_contiguous()
             and values.numel() == dev.nnz):
         raise ValueError("values must be a contiguous float32 CUDA tensor of nnz elements")
     if out is None:
         out = torch.empty((dev.ndims[mode], R), dtype=torch.float32, device=facs[0].device)
+    _check_out(out, dev.ndims[mode], R)
     ptrs = (_vp * (N + 1))(*[f.data_ptr() for f in facs], out.data_ptr())
     check(lib().ptiDevMTTKRPCOOValues(dev.handle, _vp(values.data_ptr()), ptrs, R, R, _order(mode, N, None),
                                       mode, _stream_ptr()), "ptiDevMTTKRPCOOValues")

@@ -13,7 +13,7 @@ from typing import List
 import torch
 
 from ._lib import check, lib
-from .device import DevCOO, DevHiCOO, _check_factor, _stream_ptr
+from .device import DevCOO, DevHiCOO, _check_factor, _check_factors, _stream_ptr
 
 _vp = C.c_void_p
 
@@ -102,9 +102,7 @@ def cp_als(tensor, R, niters=50, tol=1e-5, factors=None, init="uniform", seed=0,
     iterations' MTTKRPs gather BF16 copies of the factors."""
     ndims = tensor.ndims
     facs = factors if factors is not None else init_factors(ndims, R, init, seed)
-    for i, f in enumerate(facs):
-        _check_factor(f, torch.float32, "factor %d" % i)
-    stride = facs[0].shape[1]
+    stride = _check_factors(facs, ndims, torch.float32, R)
     lam = torch.empty(R, dtype=torch.float64, device=facs[0].device)
     fit = C.c_double()
     trace = (C.c_double * max(niters, 1))()

@@ -230,8 +230,8 @@ int run_coo(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, con
     const uint32_t N = t->nmodes;
     if (mode >= N) PTI_FAIL(PTIERR_VALUE_ERROR, M, "mode out of range");
     if (R == 0 || stride < R) PTI_FAIL(PTIERR_SHAPE_MISMATCH, M, "bad rank or stride");
-    for (uint32_t i = 1; i < N; ++i)
-        if (mats_order[i] >= N) PTI_FAIL(PTIERR_VALUE_ERROR, M, "mats_order entry out of range");
+    if (!order_is_permutation(N, mats_order, mode))
+        PTI_FAIL(PTIERR_VALUE_ERROR, M, "mats_order[1..N-1] must be a permutation of the other modes");
     PTI_CHECK(prepare_mode(t, mode, s), M);
     float *out = dmats[N];
     PTI_HIP(zero_floats(out, (uint64_t)t->ndims[mode] * stride, s), M);
@@ -275,7 +275,10 @@ int run_coo(ptiDevCOO *t, float *const dmats[], ptiIndex R, ptiIndex stride, con
             return 0;
         }
     }
-    const ptiIndex tile = 256;
+    // Column tiles of 64 lanes x V columns, V being the vector width the full
+    // rank and stride allow: every tile then has a lane shape (G <= 64), and
+    // tiles start at multiples of 64 columns, which keeps any V's alignment.
+    const ptiIndex tile = 64u * (ptiIndex)vector_width(R, stride, stride);
     for (ptiIndex c0 = 0; c0 < R; c0 += tile) {
         const ptiIndex Rt = std::min<ptiIndex>(tile, R - c0);
         Shape sh;
@@ -389,6 +392,42 @@ int completion_loss_grad(ptiDevCOO *t, float *const dfacs[], float *const dgrads
     return 0;
 }
 
+// Workgroups of completion_forward_kernel for nnz entries (one FP64 partial
+// sum each).
+uint32_t completion_forward_workgroups(uint64_t nnz) {
+    const uint64_t nchunks = (nnz + CHUNK - 1) / CHUNK;
+    return (uint32_t)((nchunks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
+}
+
+// CP-completion forward with the per-workgroup partial sums in `partials`
+// (completion_forward_workgroups(nnz) doubles, supplied by the caller):
+// residuals into dresid, 0.5 * sum of their squares into *dloss.  Ranks
+// wider than 64 lanes x V run each lane group over column blocks.
+int completion_forward(const ptiDevCOO *t, float *const dfacs[], ptiIndex R, ptiIndex stride, float *dresid,
+                       double *dloss, double *partials, hipStream_t s, const char *M) {
+    const uint32_t N = t->nmodes;
+    if (R == 0 || R > 256 || stride < R) PTI_FAIL(PTIERR_VALUE_ERROR, M, "rank must be in [1, 256] and <= stride");
+    Shape sh;
+    sh.V = vector_width(R, stride, stride);
+    sh.G = 4;
+    while ((uint32_t)(sh.G * sh.V) < R && sh.G < 64) sh.G <<= 1;
+    COOArgs c = {};
+    for (uint32_t m = 0; m < N; ++m) c.inds[m] = t->inds[m].as<uint32_t>();
+    c.vals = t->vals.as<float>();
+    c.nnz = t->nnz;
+    c.nchunks = (uint32_t)((t->nnz + CHUNK - 1) / CHUNK);
+    const uint32_t nblk = completion_forward_workgroups(t->nnz);
+    FactorArgs a = {};
+    for (uint32_t m = 0; m < N; ++m) a.fac[m] = dfacs[m];
+    a.R = R;
+    a.stride = stride;
+    a.out_stride = stride;
+    PTI_HIP(launch_completion(N, c, a, sh, dresid, partials, s), M);
+    hipLaunchKernelGGL(half_sum_kernel, dim3(1), dim3(256), 0, s, partials, nblk, dloss);
+    PTI_HIP(hipGetLastError(), M);
+    return 0;
+}
+
 void coo_view(const ptiDevCOO *t, uint32_t *nmodes, const ptiIndex **ndims, uint64_t *nnz, const float **vals) {
     *nmodes = t->nmodes;
     *ndims = t->ndims;
@@ -448,28 +487,12 @@ int ptiDevCOOCompletionGrad(ptiDevCOO *t, float const *dresid, float *const dmat
 int ptiDevCOOCompletionForward(ptiDevCOO *t, float *const dfacs[], ptiIndex R, ptiIndex stride, float *dresid,
                                double *dloss, void *stream) {
     static const char *M = "DevCOO Completion";
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const uint32_t N = t->nmodes;
-    Shape sh;
-    if (R == 0 || stride < R || !choose_shape(R, stride, stride, sh))
-        PTI_FAIL(PTIERR_VALUE_ERROR, M, "rank must be in [1, 256] and <= stride");
-    COOArgs c = {};
-    for (uint32_t m = 0; m < N; ++m) c.inds[m] = t->inds[m].as<uint32_t>();
-    c.vals = t->vals.as<float>();
-    c.nnz = t->nnz;
-    c.nchunks = (uint32_t)((t->nnz + CHUNK - 1) / CHUNK);
-    const uint32_t nblk = (c.nchunks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+    if (R == 0 || R > 256 || stride < R) PTI_FAIL(PTIERR_VALUE_ERROR, M, "rank must be in [1, 256] and <= stride");
+    const uint32_t nblk = completion_forward_workgroups(t->nnz);
     if (t->partials.bytes < (size_t)std::max<uint32_t>(nblk, 1) * sizeof(double))
         PTI_HIP(t->partials.alloc((size_t)std::max<uint32_t>(nblk, 1) * sizeof(double)), M);
-    FactorArgs a = {};
-    for (uint32_t m = 0; m < N; ++m) a.fac[m] = dfacs[m];
-    a.R = R;
-    a.stride = stride;
-    a.out_stride = stride;
-    PTI_HIP(launch_completion(N, c, a, sh, dresid, t->partials.as<double>(), s), M);
-    hipLaunchKernelGGL(half_sum_kernel, dim3(1), dim3(256), 0, s, t->partials.as<double>(), nblk, dloss);
-    PTI_HIP(hipGetLastError(), M);
-    return 0;
+    return completion_forward(t, dfacs, R, stride, dresid, dloss, t->partials.as<double>(),
+                              static_cast<hipStream_t>(stream), M);
 }
 
 int ptiRegisterSparseTensor(ptiSparseTensor const *X) {

@@ -6,11 +6,26 @@
 namespace pti {
 namespace dev {
 
+// mats_order[1..N-1] must name every mode other than `mode` exactly once.
+// The reference reads whatever modes it names (src/sptensor/mttkrp.c:84-97),
+// so a repeated entry there squares one factor and drops another; the
+// kernels here form the product over all non-target modes and size-check the
+// factors they gather, so such an order is rejected instead.
+inline bool order_is_permutation(ptiIndex nmodes, const ptiIndex mats_order[], ptiIndex mode) {
+    uint32_t seen = 1u << mode;
+    for (ptiIndex i = 1; i < nmodes; ++i) {
+        const ptiIndex m = mats_order[i];
+        if (m >= nmodes || (seen >> m & 1u)) return false;
+        seen |= 1u << m;
+    }
+    return true;
+}
+
 // Reference shape rules for MTTKRP arguments (src/sptensor/mttkrp.c:58-65):
 // every factor has the output's column count and ndims[m] rows.  Also checks
-// that mats_order entries are valid modes, that all matrices share one row
-// stride (the reference silently assumes it) and that the output can hold
-// the rows.
+// that mats_order[1..N-1] is a permutation of the other modes, that all
+// matrices share one row stride (the reference silently assumes it) and that
+// the output can hold the rows.
 inline int check_mttkrp_args(ptiIndex nmodes, const ptiIndex *ndims, ptiMatrix *const mats[],
                              const ptiIndex mats_order[], ptiIndex mode, const char *module) {
     if (nmodes < 2 || nmodes > 8) PTI_FAIL(PTIERR_VALUE_ERROR, module, "nmodes must be in [2, 8]");
@@ -23,8 +38,8 @@ inline int check_mttkrp_args(ptiIndex nmodes, const ptiIndex *ndims, ptiMatrix *
     }
     if (mats[nmodes]->stride != mats[0]->stride) PTI_FAIL(PTIERR_SHAPE_MISMATCH, module, "output stride differs");
     if (mats[nmodes]->cap < mats[mode]->nrows) PTI_FAIL(PTIERR_SHAPE_MISMATCH, module, "output has too few rows");
-    for (ptiIndex i = 1; i < nmodes; ++i)
-        if (mats_order[i] >= nmodes) PTI_FAIL(PTIERR_VALUE_ERROR, module, "mats_order entry out of range");
+    if (!order_is_permutation(nmodes, mats_order, mode))
+        PTI_FAIL(PTIERR_VALUE_ERROR, module, "mats_order[1..N-1] must be a permutation of the other modes");
     return 0;
 }
 

@@ -52,11 +52,20 @@ struct ptiDevHiCOOMode {
     pti::dev::Buffer binds, einds, vals, lblk, chunk_z, chunk_blk, unit_chunk, unit_br, unit_excl;
     uint32_t nchunks = 0, nunits = 0;
     bool ready = false;
-    // Stream form of the copy (16 bytes per chunk position and plane), built
-    // on first use for the lane-group count `stream_ng` of the rank shape.
+    // Stream forms of the copy (16 bytes per chunk position and plane), each
+    // built on first use for the lane-group count `ng` of a rank shape.  Two
+    // are kept, so that the FP32 and the wider BF16 lane shape of one rank
+    // (CP-ALS alternates them) do not rebuild each other; a third shape
+    // replaces the least recently used one.
+    struct StreamForm {
+        pti::dev::Buffer ent;
+        uint32_t ng = 0;
+        uint64_t last_use = 0;
+    };
     mutable std::mutex stream_mu;
-    mutable pti::dev::Buffer stream, unit_order;   // unit_order: units by decreasing chunk count
-    mutable uint32_t stream_ng = 0;
+    mutable StreamForm forms[2];
+    mutable uint64_t use_clock = 0;
+    mutable pti::dev::Buffer unit_order;   // units by decreasing chunk count
 };
 
 struct ptiDevHiCOO {
@@ -598,17 +607,27 @@ int upload(ptiDevHiCOO **out, const ptiSparseTensorHiCOO *hitsr, uint32_t mode_m
 }  // namespace
 
 namespace {
-// Builds (or rebuilds, for another lane-group count) the stream form of the
-// order-3 copy of `mode`.
-int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t s, const char *M) {
+// The stream form of the copy of `mode` for ng lane groups (*ent): a cached
+// one, or one built now in the free or least recently used slot.
+int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t s, const char *M,
+                  const uint4 **ent) {
     const ptiDevHiCOOMode &c = t->mc[mode];
     std::lock_guard<std::mutex> lk(c.stream_mu);
-    if (c.stream_ng == ng) return 0;
-    // Work queued on the old form must finish before it is released.
-    if (c.stream.ptr) PTI_HIP(hipDeviceSynchronize(), M);
+    for (auto &f : c.forms)
+        if (f.ng == ng) {
+            f.last_use = ++c.use_clock;
+            *ent = f.ent.as<uint4>();
+            return 0;
+        }
+    ptiDevHiCOOMode::StreamForm &dst =
+        c.forms[0].ng == 0 ? c.forms[0]
+                           : (c.forms[1].ng == 0 || c.forms[1].last_use < c.forms[0].last_use ? c.forms[1] : c.forms[0]);
+    // Work queued on the form being replaced must finish before it is released.
+    if (dst.ent.ptr) PTI_HIP(hipDeviceSynchronize(), M);
+    dst.ng = 0;
     const uint32_t N = t->nmodes;
     const uint32_t NP = (N + 4) / 4;
-    PTI_HIP(c.stream.alloc((size_t)c.nchunks * NP * CHUNK * sizeof(uint4)), M);
+    PTI_HIP(dst.ent.alloc((size_t)c.nchunks * NP * CHUNK * sizeof(uint4)), M);
     // Inside a row, nonzeros follow the coordinate of the largest non-target
     // mode, whose factor rows are the likeliest cache misses.
     uint32_t sec = mode == 0 ? 1 : 0;
@@ -637,12 +656,14 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
                                       spos.as<uint32_t>(), (size_t)nnz, 0, kbits, s),
             M);
     hipLaunchKernelGGL(stream_fill_kernel, grid, dim3(256), 0, s, hicoo_args(t, mode), c.nchunks, N, mode, ng,
-                       spos.as<uint32_t>(), blk_of.as<uint32_t>(), c.stream.as<uint4>());
+                       spos.as<uint32_t>(), blk_of.as<uint32_t>(), dst.ent.as<uint4>());
     PTI_HIP(hipGetLastError(), M);
-    PTI_CHECK(build_unit_order(c.unit_chunk.as<uint32_t>(), c.nunits, c.unit_order, s, M), M);
+    if (!c.unit_order.ptr) PTI_CHECK(build_unit_order(c.unit_chunk.as<uint32_t>(), c.nunits, c.unit_order, s, M), M);
     // The temporaries are released on return: the queued work must finish.
     PTI_HIP(hipStreamSynchronize(s), M);
-    c.stream_ng = ng;
+    dst.ng = ng;
+    dst.last_use = ++c.use_clock;
+    *ent = dst.ent.as<uint4>();
     return 0;
 }
 }  // namespace
@@ -656,8 +677,8 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
     const uint32_t N = t->nmodes;
     if (mode >= N) PTI_FAIL(PTIERR_VALUE_ERROR, M, "mode out of range");
     if (R == 0 || stride < R || out_stride < R) PTI_FAIL(PTIERR_SHAPE_MISMATCH, M, "bad rank or stride");
-    for (uint32_t i = 1; i < N; ++i)
-        if (mats_order[i] >= N) PTI_FAIL(PTIERR_VALUE_ERROR, M, "mats_order entry out of range");
+    if (!order_is_permutation(N, mats_order, mode))
+        PTI_FAIL(PTIERR_VALUE_ERROR, M, "mats_order[1..N-1] must be a permutation of the other modes");
     if (t->nnz && !t->mc[mode].ready) PTI_FAIL(PTIERR_VALUE_ERROR, M, "mode was not uploaded");
     PTI_HIP(zero_floats(out, (uint64_t)t->ndims[mode] * out_stride, s), M);
     if (t->nnz == 0) return 0;
@@ -665,6 +686,9 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
     // would otherwise push a workgroup past 64 KB of LDS.
     const ptiIndex tile = N >= 6 ? 32 : 64;
     const size_t esz = bf16 ? 2 : 4;
+    bool fits = true;   // the stream kernel gathers with 32-bit byte offsets
+    for (uint32_t i = 1; i < N; ++i)
+        if ((uint64_t)t->ndims[mats_order[i]] * stride * esz >= (1ull << 32)) fits = false;
     Shape first{0, 0};
     for (ptiIndex c0 = 0; c0 < R; c0 += tile) {
         const ptiIndex Rt = std::min<ptiIndex>(tile, R - c0);
@@ -672,13 +696,32 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
         // alignment the vector width V relies on.
         Shape sh;
         if (!choose_shape(Rt, stride, out_stride, sh)) PTI_FAIL(PTIERR_VALUE_ERROR, M, "unsupported rank tile");
-        // A narrower last tile keeps the first tile's lane shape when its ranks
-        // fit it (idle lanes past the rank), so every tile reads the same
-        // stream form instead of rebuilding it for another lane-group count.
-        if (c0 == 0)
+        // BF16 factors on the stream kernel: eight ranks (16 bytes) per lane
+        // when the rank tile fills at least four lanes that way, so that one
+        // gather instruction covers twice the rows of the 8-byte shape.
+        if (bf16 && fits && Rt % 8 == 0 && stride % 8 == 0 && Rt >= 32) {
+            Shape w{8, Rt / 8 <= 4 ? 4 : 8};
+            if (stream_supported(N, w.G)) sh = w;
+        }
+        // A narrower last tile keeps the first tile's lane-group width G, with
+        // the widest V <= the first tile's V that divides its ranks, when its
+        // ranks fit G * V (idle lanes past the rank): every tile then reads the
+        // same stream form instead of rebuilding it for another lane-group
+        // count.  A last tile that fits no such shape runs the tile kernel.
+        bool keep_form = true;
+        if (c0 == 0) {
             first = sh;
-        else if (Rt % first.V == 0 && Rt <= (ptiIndex)(first.G * first.V))
-            sh = first;
+        } else {
+            int v = first.V;
+            while (v > 1 && Rt % (ptiIndex)v) v >>= 1;
+            if (Rt <= (ptiIndex)(first.G * v))
+                sh = Shape{v, first.G};
+            else
+                keep_form = sh.G == first.G;
+        }
+        const bool use_stream = fits && stream_supported(N, sh.G) && keep_form;
+        // The tile kernel takes at most four ranks per lane.
+        if (!use_stream && sh.V == 8) choose_shape(Rt, stride, out_stride, sh);
         // The kernel stages factor-row offsets as 32-bit counts of V-wide vectors.
         for (uint32_t i = 1; i < N; ++i)
             if ((uint64_t)t->ndims[mats_order[i]] * (stride / sh.V) >= (1ull << 32))
@@ -691,14 +734,12 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
         a.out_stride = out_stride;
         a.mode = mode;
         for (uint32_t i = 0; i < N; ++i) a.order[i] = mats_order[i];
-        bool fits = true;   // the stream kernel gathers with 32-bit byte offsets
-        for (uint32_t i = 1; i < N; ++i)
-            if ((uint64_t)t->ndims[mats_order[i]] * stride * esz >= (1ull << 32)) fits = false;
-        if (fits && stream_supported(N, sh.G)) {
-            PTI_CHECK(ensure_stream(t, mode, 64u / sh.G, s, M), M);
+        if (use_stream) {
+            const uint4 *ent = nullptr;
+            PTI_CHECK(ensure_stream(t, mode, 64u / sh.G, s, M, &ent), M);
             const ptiDevHiCOOMode &c = t->mc[mode];
             StreamArgs h{};
-            h.ent = c.stream.as<uint4>();
+            h.ent = ent;
             h.unit_chunk = c.unit_chunk.as<uint32_t>();
             h.unit_br = c.unit_br.as<uint32_t>();
             h.unit_excl = c.unit_excl.as<uint32_t>();
@@ -854,7 +895,7 @@ int ptiDevHiCOOInfo(ptiDevHiCOO const *t, uint64_t *nnz, uint64_t *nblocks, uint
         *bytes = t->bytes;
         for (uint32_t m = 0; m < t->nmodes; ++m) {
             std::lock_guard<std::mutex> lk(t->mc[m].stream_mu);
-            *bytes += t->mc[m].stream.bytes;
+            for (const auto &f : t->mc[m].forms) *bytes += f.ent.bytes;
         }
     }
     return 0;

@@ -17,17 +17,21 @@ struct Shape {
     int G;
 };
 
-// V = 4 when R and every stride are multiples of 4 (16-byte row pieces),
-// else 2, else 1; G = ceil(R / V) rounded up to a power of two in [4, 64].
+// Ranks per lane: 4 when R and every stride are multiples of 4 (16-byte row
+// pieces), else 2 when they are even, else 1.
+inline int vector_width(uint32_t R, uint32_t stride, uint32_t out_stride) {
+    if (R % 4 == 0 && stride % 4 == 0 && out_stride % 4 == 0) return 4;
+    if (R % 2 == 0 && stride % 2 == 0 && out_stride % 2 == 0) return 2;
+    return 1;
+}
+
+// V = vector_width; G = ceil(R / V) rounded up to a power of two in [4, 64].
 // Returns false when R exceeds 64 * V; callers split larger ranks into
-// column tiles (256 columns for COO, 64 for the HiCOO LDS tile).
+// column tiles (64 * V columns for COO, 64 for the HiCOO LDS tile) or loop
+// over column blocks (CP-completion forward).
 inline bool choose_shape(uint32_t R, uint32_t stride, uint32_t out_stride, Shape &sh) {
     if (R == 0) return false;
-    int V = 1;
-    if (R % 4 == 0 && stride % 4 == 0 && out_stride % 4 == 0)
-        V = 4;
-    else if (R % 2 == 0 && stride % 2 == 0 && out_stride % 2 == 0)
-        V = 2;
+    const int V = vector_width(R, stride, out_stride);
     uint32_t lanes = (R + V - 1) / V;
     int G = 4;
     while ((uint32_t)G < lanes) G <<= 1;

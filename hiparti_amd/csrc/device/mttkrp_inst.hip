@@ -65,10 +65,16 @@ hipError_t stream_vg(const StreamArgs &h, const FactorArgs &a, bool bf16, hipStr
         if (h.nunits == 0) return hipSuccess;
         const size_t lds = (size_t)(1u << h.sb_bits) * (G * V + 1) * sizeof(float);
         dim3 grid(h.nunits);
-        if (bf16)
+        if constexpr (V == 8) {
+            // Eight ranks per lane exist only for BF16 factors (16-byte pieces).
+            if (!bf16) return hipErrorInvalidValue;
             hipLaunchKernelGGL((hicoo_stream_kernel<N, V, G, uint16_t, 8>), grid, dim3(512), lds, s, h, a);
-        else
-            hipLaunchKernelGGL((hicoo_stream_kernel<N, V, G, float, 8>), grid, dim3(512), lds, s, h, a);
+        } else {
+            if (bf16)
+                hipLaunchKernelGGL((hicoo_stream_kernel<N, V, G, uint16_t, 8>), grid, dim3(512), lds, s, h, a);
+            else
+                hipLaunchKernelGGL((hicoo_stream_kernel<N, V, G, float, 8>), grid, dim3(512), lds, s, h, a);
+        }
         return hipGetLastError();
     }
 }
@@ -175,6 +181,12 @@ hipError_t PTI_CAT(launch_hicoo_, PTI_NMODES)(const HiCOOArgs &h, const FactorAr
 hipError_t PTI_CAT(launch_stream_, PTI_NMODES)(const StreamArgs &h, const FactorArgs &a, Shape sh, bool bf16,
                                                hipStream_t s) {
     switch (sh.V) {
+        case 8:
+            switch (sh.G) {
+                case 4: return stream_vg<8, 4>(h, a, bf16, s);
+                case 8: return stream_vg<8, 8>(h, a, bf16, s);
+                default: return hipErrorInvalidValue;
+            }
         case 4: return stream_v<4>(h, a, sh.G, bf16, s);
         case 2: return stream_v<2>(h, a, sh.G, bf16, s);
         case 1: return stream_v<1>(h, a, sh.G, bf16, s);

@@ -389,6 +389,7 @@ struct Vec;
 template <> struct Vec<float, 4> { using T = float4; };
 template <> struct Vec<float, 2> { using T = float2; };
 template <> struct Vec<float, 1> { using T = float; };
+template <> struct Vec<uint16_t, 8> { using T = uint4; };
 template <> struct Vec<uint16_t, 4> { using T = uint2; };
 template <> struct Vec<uint16_t, 2> { using T = uint32_t; };
 template <> struct Vec<uint16_t, 1> { using T = uint16_t; };
@@ -396,6 +397,12 @@ template <> struct Vec<uint16_t, 1> { using T = uint16_t; };
 __device__ __forceinline__ void to_float(const float4 &t, float *o) { o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w; }
 __device__ __forceinline__ void to_float(const float2 &t, float *o) { o[0] = t.x; o[1] = t.y; }
 __device__ __forceinline__ void to_float(const float &t, float *o) { o[0] = t; }
+__device__ __forceinline__ void to_float(const uint4 &t, float *o) {
+    o[0] = __uint_as_float(t.x << 16); o[1] = __uint_as_float(t.x & 0xFFFF0000u);
+    o[2] = __uint_as_float(t.y << 16); o[3] = __uint_as_float(t.y & 0xFFFF0000u);
+    o[4] = __uint_as_float(t.z << 16); o[5] = __uint_as_float(t.z & 0xFFFF0000u);
+    o[6] = __uint_as_float(t.w << 16); o[7] = __uint_as_float(t.w & 0xFFFF0000u);
+}
 __device__ __forceinline__ void to_float(const uint2 &t, float *o) {
     o[0] = __uint_as_float(t.x << 16); o[1] = __uint_as_float(t.x & 0xFFFF0000u);
     o[2] = __uint_as_float(t.y << 16); o[3] = __uint_as_float(t.y & 0xFFFF0000u);
@@ -949,8 +956,9 @@ __global__ void __launch_bounds__(256) coo_mttkrp_kernel(COOArgs c, FactorArgs a
 // Sampled reconstruction at the nonzeros (CP completion forward):
 //   yhat_z = sum_r prod_m A_m[i_m][r],  resid_z = x_z - yhat_z,
 // and each workgroup's partial sum of resid_z^2 in FP64.  Storage order, one
-// wave per chunk of CHUNK nonzeros, G lanes x V ranks per nonzero; the rank
-// sum is a butterfly over the G lanes of a group.
+// wave per chunk of CHUNK nonzeros, G lanes x V ranks per nonzero, the group
+// stepping over column blocks of G * V ranks when R is wider; the rank sum is
+// a butterfly over the G lanes of a group.
 template <int N, int V, int G>
 __global__ void __launch_bounds__(256) completion_forward_kernel(COOArgs c, FactorArgs a, float *resid,
                                                                  double *partials) {
@@ -971,17 +979,19 @@ __global__ void __launch_bounds__(256) completion_forward_kernel(COOArgs c, Fact
             const bool valid = p < cnt;
             const uint64_t z = z0 + (valid ? p : 0);
             float part = 0.f;
-            if (valid && col < a.R) {
-                float prod[V], t[V];
-                FactorLoad<float, V>::load(a.fac[0], (uint64_t)c.inds[0][z] * a.stride + col, prod);
+            if (valid) {
+                for (uint32_t cc = col; cc < a.R; cc += G * V) {
+                    float prod[V], t[V];
+                    FactorLoad<float, V>::load(a.fac[0], (uint64_t)c.inds[0][z] * a.stride + cc, prod);
 #pragma unroll
-                for (int m = 1; m < N; ++m) {
-                    FactorLoad<float, V>::load(a.fac[m], (uint64_t)c.inds[m][z] * a.stride + col, t);
+                    for (int m = 1; m < N; ++m) {
+                        FactorLoad<float, V>::load(a.fac[m], (uint64_t)c.inds[m][z] * a.stride + cc, t);
 #pragma unroll
-                    for (int k = 0; k < V; ++k) prod[k] *= t[k];
+                        for (int k = 0; k < V; ++k) prod[k] *= t[k];
+                    }
+#pragma unroll
+                    for (int k = 0; k < V; ++k) part += prod[k];
                 }
-#pragma unroll
-                for (int k = 0; k < V; ++k) part += prod[k];
             }
 #pragma unroll
             for (int off = G / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);

@@ -27,9 +27,12 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cerrno>
+#include <cinttypes>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 
 namespace {
 
@@ -559,6 +562,40 @@ int ptiSparseTensorToHiCOO(ptiSparseTensorHiCOO *hitsr, ptiNnzIndex *max_nnzb, p
     uint64_t mx_nnzb = hitsr->bptr.data[1] - hitsr->bptr.data[0];
     for (uint64_t b = 0; b < nb; ++b) mx_nnzb = std::max(mx_nnzb, hitsr->bptr.data[b + 1] - hitsr->bptr.data[b]);
     if (max_nnzb) *max_nnzb = mx_nnzb;
+    return 0;
+}
+
+// Text dump of a HiCOO tensor in the reference's layout (src/sptensor/hicoo/
+// dump.c:30-74): order and sizes, nkiters, the superblock-row schedule of
+// every mode, kptr, cptr, bptr, the block coordinates, the element offsets
+// and the values, each array in the vector dump format.
+int ptiDumpSparseTensorHiCOO(ptiSparseTensorHiCOO *const hitsr, FILE *fp) {
+    static const char *M = "HiSpTns Dump";
+    const ptiIndex N = hitsr->nmodes;
+    if (std::fprintf(fp, "%" PRIu32 "\n", N) < 0) PTI_FAIL(PTIERR_OS_ERROR + errno, M, std::strerror(errno));
+    for (ptiIndex m = 0; m < N; ++m) std::fprintf(fp, m ? " %" PRIu32 : "%" PRIu32, hitsr->ndims[m]);
+    std::fputs("\n", fp);
+    std::fprintf(fp, "nkiters:\n");
+    PTI_CHECK(ptiDumpIndexArray(hitsr->nkiters, N, fp), M);
+    std::fprintf(fp, "kschr:\n");
+    const uint64_t sk = 1ull << hitsr->sk_bits;
+    for (ptiIndex m = 0; m < N; ++m) {
+        std::fprintf(fp, "mode %" PRIu32 "\n", m);
+        const uint64_t nrows = (hitsr->ndims[m] + sk - 1) / sk;
+        for (uint64_t i = 0; i < nrows; ++i) PTI_CHECK(ptiDumpIndexVector(&hitsr->kschr[m][i], fp), M);
+    }
+    std::fprintf(fp, "kptr:\n");
+    PTI_CHECK(ptiDumpNnzIndexVector(&hitsr->kptr, fp), M);
+    std::fprintf(fp, "cptr:\n");
+    PTI_CHECK(ptiDumpNnzIndexVector(&hitsr->cptr, fp), M);
+    std::fprintf(fp, "bptr:\n");
+    PTI_CHECK(ptiDumpNnzIndexVector(&hitsr->bptr, fp), M);
+    std::fprintf(fp, "binds:\n");
+    for (ptiIndex m = 0; m < N; ++m) PTI_CHECK(ptiDumpBlockIndexVector(&hitsr->binds[m], fp), M);
+    std::fprintf(fp, "einds:\n");
+    for (ptiIndex m = 0; m < N; ++m) PTI_CHECK(ptiDumpElementIndexVector(&hitsr->einds[m], fp), M);
+    std::fprintf(fp, "values:\n");
+    PTI_CHECK(ptiDumpValueVector(&hitsr->values, fp), M);
     return 0;
 }
 

@@ -5,7 +5,11 @@
 // vector data the way reference-linked code does.
 #include "internal.h"
 
+#include <cerrno>
+#include <cinttypes>
+#include <cstdio>
 #include <cstdlib>
+#include <cstring>
 
 namespace {
 
@@ -114,6 +118,51 @@ int ptiResizeNnzIndexVector(ptiNnzIndexVector *v, ptiNnzIndex const size) {
     return vec_resize<ptiNnzIndexVector, ptiNnzIndex>(v, size, "NnzIdxVec Resize");
 }
 void ptiFreeNnzIndexVector(ptiNnzIndexVector *v) { vec_free(v); }
+
+// Text dumps (src/vector/dump.c:31-175): a "<type> length: n" line, then
+// the entries, each followed by a tab, then a newline.
+}  // extern "C"
+
+namespace {
+template <typename T>
+int dump_entries(const char *title, const T *data, uint64_t n, const char *fmt, const char *module, FILE *fp) {
+    if (std::fprintf(fp, "%s length: %" PRIu64 "\n", title, n) < 0)
+        PTI_FAIL(PTIERR_OS_ERROR + errno, module, std::strerror(errno));
+    for (uint64_t i = 0; i < n; ++i)
+        if (std::fprintf(fp, fmt, data[i]) < 0) PTI_FAIL(PTIERR_OS_ERROR + errno, module, std::strerror(errno));
+    std::fprintf(fp, "\n");
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int ptiDumpElementIndexVector(ptiElementIndexVector *vec, FILE *fp) {
+    return dump_entries("ptiElementIndexVector", vec->data, vec->len, "%" PRIu8 "\t", "EleIdxVec Dump", fp);
+}
+int ptiDumpIndexVector(ptiIndexVector *vec, FILE *fp) {
+    return dump_entries("ptiIndexVector", vec->data, vec->len, "%" PRIu32 "\t", "IdxVec Dump", fp);
+}
+int ptiDumpBlockIndexVector(ptiBlockIndexVector *vec, FILE *fp) {
+    return dump_entries("ptiBlockIndexVector", vec->data, vec->len, "%" PRIu32 "\t", "BlkIdxVec Dump", fp);
+}
+int ptiDumpNnzIndexVector(ptiNnzIndexVector *vec, FILE *fp) {
+    return dump_entries("ptiNnzIndexVector", vec->data, vec->len, "%" PRIu64 "\t", "NnzIdxVec Dump", fp);
+}
+int ptiDumpValueVector(ptiValueVector *vec, FILE *fp) {
+    const ptiValue *d = vec->data;
+    if (std::fprintf(fp, "ptiValueVector length: %" PRIu64 "\n", vec->len) < 0)
+        PTI_FAIL(PTIERR_OS_ERROR + errno, "ValVec Dump", std::strerror(errno));
+    for (uint64_t i = 0; i < vec->len; ++i) std::fprintf(fp, "%f\t", (double)d[i]);
+    std::fprintf(fp, "\n");
+    return 0;
+}
+int ptiDumpIndexArray(ptiIndex const *array, ptiNnzIndex const n, FILE *fp) {
+    return dump_entries("ptiIndex array", array, n, "%" PRIu32 "\t", "IdxArray Dump", fp);
+}
+int ptiDumpNnzIndexArray(ptiNnzIndex const *array, ptiNnzIndex const n, FILE *fp) {
+    return dump_entries("ptiNnzIndex array", array, n, "%" PRIu64 "\t", "NnzIdxArray Dump", fp);
+}
 
 ptiIndex ptiMaxIndexArray(ptiIndex const *const indices, ptiNnzIndex const size) {
     ptiIndex m = indices[0];

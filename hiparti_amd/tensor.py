@@ -292,6 +292,13 @@ class HiCOOTensor:
         """Copies of the HiCOO arrays (see hicoo_struct_arrays)."""
         return hicoo_struct_arrays(self.address)
 
+    def dump(self, filename):
+        """Text dump of every HiCOO array (ptiDumpSparseTensorHiCOO,
+        HiParTIPy/Tensor.py:156-158, which opens the file for reading; here it
+        is opened for writing)."""
+        with _CFile(filename, "w") as fp:
+            check(lib().ptiDumpSparseTensorHiCOO(C.byref(self.address), fp), "ptiDumpSparseTensorHiCOO")
+
     def cpd(self, rank, niters=50, tol=1e-5):
         """ptiCpdAlsHiCOO (reference src/sptensor/hicoo/cpd.c:153-204) on the
         GPU, rank <= 248: returns (lambda, factors, fit)."""

@@ -210,6 +210,14 @@ int  ptiAppendNnzIndexVector(ptiNnzIndexVector *vec, ptiNnzIndex const value);
 int  ptiResizeNnzIndexVector(ptiNnzIndexVector *vec, ptiNnzIndex const size);
 void ptiFreeNnzIndexVector(ptiNnzIndexVector *vec);
 ptiIndex ptiMaxIndexArray(ptiIndex const * const indices, ptiNnzIndex const size);
+/* Text dumps of vectors and arrays (reference: vectors.h, src/vector/dump.c:31-175). */
+int ptiDumpElementIndexVector(ptiElementIndexVector *vec, FILE *fp);
+int ptiDumpIndexVector(ptiIndexVector *vec, FILE *fp);
+int ptiDumpBlockIndexVector(ptiBlockIndexVector *vec, FILE *fp);
+int ptiDumpNnzIndexVector(ptiNnzIndexVector *vec, FILE *fp);
+int ptiDumpValueVector(ptiValueVector *vec, FILE *fp);
+int ptiDumpIndexArray(ptiIndex const *array, ptiNnzIndex const n, FILE *fp);
+int ptiDumpNnzIndexArray(ptiNnzIndex const *array, ptiNnzIndex const n, FILE *fp);
 
 /* ---------------------------------------------------------------------------
  * Dense matrices (reference: include/includes/matrices.h:26-65).
@@ -286,6 +294,8 @@ int  ptiSparseTensorToHiCOO(ptiSparseTensorHiCOO *hitsr, ptiNnzIndex *max_nnzb,
                             const ptiElementIndex sk_bits, const ptiElementIndex sc_bits,
                             int const tk);
 double SparseTensorFrobeniusNormSquaredHiCOO(ptiSparseTensorHiCOO const * const hitsr);
+/* Text dump of every HiCOO array (reference: sptensors.h:128, src/sptensor/hicoo/dump.c:30-74). */
+int  ptiDumpSparseTensorHiCOO(ptiSparseTensorHiCOO * const hitsr, FILE *fp);
 
 /* ---------------------------------------------------------------------------
  * MTTKRP (the hot path).

@@ -77,12 +77,25 @@ def kruskal_fit(normX2, grams, lam, last, M):
     return 1 - residual / np.sqrt(normX2)
 
 
-def cp_als(inds, vals, facs, niters, tol, round_factors=np.float32):
+def to_bf16(a):
+    """Round to the nearest bfloat16 (ties to even), returned as float64: the
+    rounding of the device's BF16 factor copies."""
+    x = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    x = (x + 0x7FFF + ((x >> 16) & 1)) & 0xFFFF0000
+    return x.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def cp_als(inds, vals, facs, niters, tol, round_factors=np.float32, bf16_iters=0, mttkrp=None):
     """Run the reference's ALS from initial factors `facs` (list of arrays).
     With round_factors=np.float32 each updated factor is rounded to FP32, as
-    FP32 factor storage does; None keeps FP64 throughout.
+    FP32 factor storage does; None keeps FP64 throughout.  During the first
+    bf16_iters iterations the MTTKRPs read bfloat16-rounded copies of the
+    factors (the device's BF16-then-FP32 schedule; the solve, the norms and
+    the fit are unchanged).  mttkrp(inds, vals, factors, mode) replaces the
+    numpy FP64 MTTKRP (e.g. the C oracle's, for large tensors).
     Returns (lambda, factors, fits)."""
     N = len(facs)
+    mttkrp = mttkrp or mttkrp_f64
     A = [np.asarray(f, np.float64) for f in facs]
     G = [a.T @ a for a in A]
     normX2 = float(np.dot(np.asarray(vals, np.float64), np.asarray(vals, np.float64)))
@@ -92,7 +105,7 @@ def cp_als(inds, vals, facs, niters, tol, round_factors=np.float32):
     M = None
     for it in range(niters):
         for m in range(N):
-            M = mttkrp_f64(inds, vals, A, m)
+            M = mttkrp(inds, vals, [to_bf16(a) for a in A] if it < bf16_iters else A, m)
             if round_factors is not None:
                 M = M.astype(round_factors).astype(np.float64)
             a, _ = solve_normals(G, m, M)

@@ -59,6 +59,15 @@ def lib():
         L.orc_hicoo_convert.restype = C.c_int
         L.orc_hicoo_free.argtypes = [_P(OrcHiCOO)]
         L.orc_sort_lexicographic.argtypes = [C.c_uint32, C.c_uint64, _P(_u32p), _f32p]
+        rt = [C.c_uint32, C.c_uint64, _P(_u32p), _f32p, C.c_uint32, C.c_uint32, _P(_f32p), C.c_uint32, C.c_uint64,
+              C.c_uint32, _u32p, _f64p]
+        L.orc_mttkrp_f64_rows.argtypes = rt + [C.c_int]
+        L.orc_completion_f64.argtypes = rt + [_f64p, C.c_int]
+        L.orc_omp_cpd_als.argtypes = [C.c_uint32, C.c_uint64, _P(_u32p), _f32p, _u32p, C.c_uint32, C.c_uint32,
+                                      C.c_uint32, C.c_double, _P(_f32p), _f32p, _f64p, _u32p, _f64p, _f64p, C.c_int]
+        L.orc_omp_cpd_als.restype = C.c_double
+        L.orc_mttkrp_f64_rows.restype = None
+        L.orc_completion_f64.restype = None
         for f in ("orc_mttkrp_f64", "orc_mttkrp_coo_seq", "orc_omp_mttkrp_coo", "orc_mttkrp_hicoo_seq",
                   "orc_omp_mttkrp_hicoo", "orc_hicoo_free", "orc_sort_lexicographic"):
             getattr(L, f).restype = None
@@ -107,6 +116,61 @@ def mttkrp(inds, vals, facs, mode, order=None, nrows_out=None, kind="f64", nthre
     else:
         raise ValueError(kind)
     return out
+
+
+def _row_args(inds, vals, facs, mode, rows):
+    inds = np.ascontiguousarray(inds, np.uint32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    N, nnz = inds.shape
+    R = facs[0].shape[1]
+    fs = [np.ascontiguousarray(f, np.float32) for f in facs]
+    stride = fs[0].shape[1]
+    rows = np.ascontiguousarray(rows if rows is not None else np.zeros(0), np.uint32)
+    keep = (inds, vals, fs, rows)
+    args = [N, nnz, _ptr_array([inds[m] for m in range(N)], C.c_uint32), vals.ctypes.data_as(_f32p), R, stride,
+            _ptr_array(fs, C.c_float), mode, fs[mode].shape[0], len(rows), rows.ctypes.data_as(_u32p)]
+    return args, keep, R
+
+
+def mttkrp_rows(inds, vals, facs, mode, rows, nthreads=0):
+    """FP64 MTTKRP restricted to output rows `rows` (len(rows) x R), for
+    accuracy checks on tensors too large for the full FP64 closed form.
+    Factors are used with their own row stride (all must share it)."""
+    args, keep, R = _row_args(inds, vals, facs, mode, rows)
+    out = np.zeros((len(keep[3]), R), np.float64)
+    lib().orc_mttkrp_f64_rows(*args, out.ctypes.data_as(_f64p), int(nthreads))
+    return out
+
+
+def completion_f64(inds, vals, facs, mode=0, rows=None, nthreads=0):
+    """FP64 CP-completion loss 0.5 sum_z e_z^2 and, for output rows `rows` of
+    `mode`, the MTTKRP of the residuals e_z (-dL/dA_mode there).  Returns
+    (loss, rows x R array)."""
+    args, keep, R = _row_args(inds, vals, facs, mode, rows)
+    out = np.zeros((len(keep[3]), R), np.float64)
+    loss = C.c_double()
+    lib().orc_completion_f64(*args, out.ctypes.data_as(_f64p), C.byref(loss), int(nthreads))
+    return loss.value, out
+
+
+def omp_cpd_als(inds, vals, ndims, facs, niters, tol=0.0, nthreads=0):
+    """Host CP-ALS (orc_omp_cpd_als, the restatement of ptiOmpCpdAls with
+    use_reduce = 0) from initial factors `facs` (float32, rows x R).
+    Returns (lambda, factors, fits, seconds in MTTKRPs, seconds elsewhere)."""
+    inds = np.ascontiguousarray(inds, np.uint32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    N, nnz = inds.shape
+    R = facs[0].shape[1]
+    fs = [np.array(f, np.float32, order="C", copy=True) for f in facs]
+    lam = np.zeros(R, np.float32)
+    trace = np.zeros(max(niters, 1), np.float64)
+    done = C.c_uint32()
+    tm, to = C.c_double(), C.c_double()
+    lib().orc_omp_cpd_als(N, nnz, _ptr_array([inds[m] for m in range(N)], C.c_uint32), vals.ctypes.data_as(_f32p),
+                          (C.c_uint32 * N)(*[int(d) for d in ndims]), R, R, niters, float(tol),
+                          _ptr_array(fs, C.c_float), lam.ctypes.data_as(_f32p), trace.ctypes.data_as(_f64p),
+                          C.byref(done), C.byref(tm), C.byref(to), int(nthreads or os.cpu_count()))
+    return lam, fs, list(trace[:done.value]), tm.value, to.value
 
 
 def mttkrp_hicoo(h, sb_bits, facs, mode, order=None, nrows_out=None, kind="seq", nthreads=0):

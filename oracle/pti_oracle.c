@@ -40,6 +40,131 @@ void orc_mttkrp_f64(uint32_t N, uint64_t nnz, const uint32_t *const *inds, const
     }
 }
 
+/* The FP64 closed form above restricted to a set of output rows, for
+ * full-size accuracy checks: out[s][r] = sum over the nonzeros z whose mode
+ * coordinate is sel[s] of v_z prod_{k != mode} A_k[i_k][r] (product in
+ * ascending mode order).  OpenMP over the nonzeros with per-thread FP64
+ * accumulators, summed in thread order.  Rows listed twice get the same sum.
+ * Pinned against orc_mttkrp_f64 on the selected rows (tests). */
+void orc_mttkrp_f64_rows(uint32_t N, uint64_t nnz, const uint32_t *const *inds, const float *vals, uint32_t R,
+                         uint32_t stride, const float *const *facs, uint32_t mode, uint64_t nrows_out,
+                         uint32_t nsel, const uint32_t *sel, double *out, int nthreads) {
+    int32_t *slot = (int32_t *)malloc((nrows_out ? nrows_out : 1) * sizeof(int32_t));
+    for (uint64_t i = 0; i < nrows_out; ++i) slot[i] = -1;
+    for (uint32_t s = 0; s < nsel; ++s)
+        if (sel[s] < nrows_out && slot[sel[s]] < 0) slot[sel[s]] = (int32_t)s;
+    memset(out, 0, (size_t)nsel * R * sizeof(double));
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+    nthreads = 1;
+#endif
+    double *acc = (double *)calloc((size_t)nthreads * nsel * R + 1, sizeof(double));
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+#ifdef _OPENMP
+        const int t = omp_get_thread_num();
+#else
+        const int t = 0;
+#endif
+        double *my = acc + (size_t)t * nsel * R;
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (uint64_t z = 0; z < nnz; ++z) {
+            const int32_t s = slot[inds[mode][z]];
+            if (s < 0) continue;
+            double *row = my + (size_t)s * R;
+            for (uint32_t r = 0; r < R; ++r) {
+                double p = vals[z];
+                for (uint32_t m = 0; m < N; ++m)
+                    if (m != mode) p *= (double)facs[m][(uint64_t)inds[m][z] * stride + r];
+                row[r] += p;
+            }
+        }
+    }
+    for (uint32_t s = 0; s < nsel; ++s) {
+        const int32_t src = slot[sel[s]];
+        for (int t = 0; t < nthreads; ++t)
+            for (uint32_t r = 0; r < R; ++r) out[(size_t)s * R + r] += acc[((size_t)t * nsel + src) * R + r];
+    }
+    free(acc);
+    free(slot);
+}
+
+/* CP-completion loss and gradient rows in FP64 (the loss has no reference
+ * counterpart; its gradient is the reference MTTKRP, src/sptensor/mttkrp.c:
+ * 39-108, of the residual tensor): with e_z = v_z - sum_r prod_m A_m[i_m][r],
+ *   *loss = 0.5 * sum_z e_z^2,
+ *   out[s][r] = sum over z with i_mode = sel[s] of e_z prod_{m != mode} A_m[i_m][r]
+ * (= -dL/dA_mode at row sel[s]).  nsel may be 0 (loss only).  OpenMP over the
+ * nonzeros, per-thread FP64 sums added in thread order.  Pinned against
+ * numpy FP64 and orc_mttkrp_f64 on the FP64 residuals (tests). */
+void orc_completion_f64(uint32_t N, uint64_t nnz, const uint32_t *const *inds, const float *vals, uint32_t R,
+                        uint32_t stride, const float *const *facs, uint32_t mode, uint64_t nrows_out,
+                        uint32_t nsel, const uint32_t *sel, double *out, double *loss, int nthreads) {
+    int32_t *slot = (int32_t *)malloc((nrows_out ? nrows_out : 1) * sizeof(int32_t));
+    for (uint64_t i = 0; i < nrows_out; ++i) slot[i] = -1;
+    for (uint32_t s = 0; s < nsel; ++s)
+        if (sel[s] < nrows_out && slot[sel[s]] < 0) slot[sel[s]] = (int32_t)s;
+    if (nsel) memset(out, 0, (size_t)nsel * R * sizeof(double));
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+    nthreads = 1;
+#endif
+    double *acc = (double *)calloc((size_t)nthreads * nsel * R + 1, sizeof(double));
+    double *lsum = (double *)calloc((size_t)nthreads, sizeof(double));
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+#ifdef _OPENMP
+        const int t = omp_get_thread_num();
+#else
+        const int t = 0;
+#endif
+        double *my = acc + (size_t)t * nsel * R;
+        double l = 0.0;
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (uint64_t z = 0; z < nnz; ++z) {
+            double yhat = 0.0;
+            for (uint32_t r = 0; r < R; ++r) {
+                double p = 1.0;
+                for (uint32_t m = 0; m < N; ++m) p *= (double)facs[m][(uint64_t)inds[m][z] * stride + r];
+                yhat += p;
+            }
+            const double e = (double)vals[z] - yhat;
+            l += e * e;
+            const int32_t s = nsel ? slot[inds[mode][z]] : -1;
+            if (s < 0) continue;
+            double *row = my + (size_t)s * R;
+            for (uint32_t r = 0; r < R; ++r) {
+                double p = e;
+                for (uint32_t m = 0; m < N; ++m)
+                    if (m != mode) p *= (double)facs[m][(uint64_t)inds[m][z] * stride + r];
+                row[r] += p;
+            }
+        }
+        lsum[t] = l;
+    }
+    double l = 0.0;
+    for (int t = 0; t < nthreads; ++t) l += lsum[t];
+    *loss = 0.5 * l;
+    for (uint32_t s = 0; s < nsel; ++s) {
+        const int32_t src = slot[sel[s]];
+        for (int t = 0; t < nthreads; ++t)
+            for (uint32_t r = 0; r < R; ++r) out[(size_t)s * R + r] += acc[((size_t)t * nsel + src) * R + r];
+    }
+    free(lsum);
+    free(acc);
+    free(slot);
+}
+
 /* src/sptensor/mttkrp.c:111-163 (3-D) and :39-108 (N-D). */
 void orc_mttkrp_coo_seq(uint32_t N, uint64_t nnz, const uint32_t *const *inds, const float *vals,
                         uint32_t R, uint32_t stride, const float *const *facs, const uint32_t *order,
@@ -112,6 +237,265 @@ void orc_omp_mttkrp_coo(uint32_t N, uint64_t nnz, const uint32_t *const *inds, c
         }
         free(scratch);
     }
+}
+
+/* ------------------------------------------------------------------------- */
+/* CP-ALS on the host (the CPU baseline of the device CP-ALS)                */
+/* ------------------------------------------------------------------------- */
+
+static double orc_wtime(void) {
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+}
+
+/* G = A^T A (R x R, FP64, both triangles) of an I x R row-major FP32 matrix:
+ * the role of ssyrk_ in the reference (src/sptensor/cpd_omp.c:75,146).
+ * OpenMP over rows with per-thread partial sums added in thread order. */
+static void orc_gram(const float *A, uint64_t I, uint32_t R, uint32_t stride, double *G, int nthreads) {
+    double *part = (double *)calloc((size_t)nthreads * R * R, sizeof(double));
+#pragma omp parallel num_threads(nthreads)
+    {
+#ifdef _OPENMP
+        double *my = part + (size_t)omp_get_thread_num() * R * R;
+#else
+        double *my = part;
+#endif
+#pragma omp for schedule(static)
+        for (uint64_t i = 0; i < I; ++i) {
+            const float *a = A + i * stride;
+            for (uint32_t p = 0; p < R; ++p) {
+                const double ap = a[p];
+                for (uint32_t q = p; q < R; ++q) my[p * R + q] += ap * a[q];
+            }
+        }
+    }
+    for (uint32_t p = 0; p < R; ++p)
+        for (uint32_t q = p; q < R; ++q) {
+            double t = 0.0;
+            for (int th = 0; th < nthreads; ++th) t += part[(size_t)th * R * R + p * R + q];
+            G[p * R + q] = G[q * R + p] = t;
+        }
+    free(part);
+}
+
+/* Rows of M (I x R, FP32) solved against V (R x R, FP64 SPD or not):
+ * A[i] = M[i] V^-1, the work of ptiMatrixSolveNormals (src/matrix/
+ * matrix_solver.c:32-83: spotrf/spotrs, on failure sgesv).  Cholesky by the
+ * Cholesky-Banachiewicz recurrence (row by row, L L^T = V); if a pivot is
+ * not positive, LU with partial pivoting by Doolittle's recurrence; if V is
+ * exactly singular the rows are left as they are (as gesv leaves its
+ * right-hand side).  Returns 0 Cholesky, 1 LU, 2 singular. */
+static int orc_solve_rows(const double *V, uint32_t R, float *A, uint64_t I, uint32_t stride, int nthreads) {
+    double *L = (double *)calloc((size_t)R * R, sizeof(double));
+    int chol = 1;
+    for (uint32_t i = 0; i < R && chol; ++i)
+        for (uint32_t j = 0; j <= i; ++j) {
+            double sum = V[i * R + j];
+            for (uint32_t k = 0; k < j; ++k) sum -= L[i * R + k] * L[j * R + k];
+            if (i == j) {
+                if (!(sum > 0.0)) { chol = 0; break; }
+                L[i * R + i] = sqrt(sum);
+            } else {
+                L[i * R + j] = sum / L[j * R + j];
+            }
+        }
+    int status = 0;
+    uint32_t *piv = NULL;
+    if (!chol) {
+        /* Doolittle LU with partial pivoting, in place in L (unit lower, upper). */
+        status = 1;
+        memcpy(L, V, (size_t)R * R * sizeof(double));
+        piv = (uint32_t *)malloc((R ? R : 1) * sizeof(uint32_t));
+        for (uint32_t i = 0; i < R; ++i) piv[i] = i;
+        for (uint32_t k = 0; k < R && status == 1; ++k) {
+            uint32_t p = k;
+            for (uint32_t i = k + 1; i < R; ++i)
+                if (fabs(L[i * R + k]) > fabs(L[p * R + k])) p = i;
+            if (L[p * R + k] == 0.0) { status = 2; break; }
+            if (p != k) {
+                for (uint32_t j = 0; j < R; ++j) {
+                    double t = L[k * R + j]; L[k * R + j] = L[p * R + j]; L[p * R + j] = t;
+                }
+                uint32_t t = piv[k]; piv[k] = piv[p]; piv[p] = t;
+            }
+            for (uint32_t i = k + 1; i < R; ++i) {
+                L[i * R + k] /= L[k * R + k];
+                for (uint32_t j = k + 1; j < R; ++j) L[i * R + j] -= L[i * R + k] * L[k * R + j];
+            }
+        }
+    }
+    if (status != 2) {
+        /* V symmetric: A[i]^T = V^-1 M[i]^T. */
+#pragma omp parallel num_threads(nthreads)
+        {
+            double *x = (double *)malloc((R ? R : 1) * sizeof(double));
+#pragma omp for schedule(static)
+            for (uint64_t i = 0; i < I; ++i) {
+                float *a = A + i * stride;
+                if (status == 0) {
+                    for (uint32_t r = 0; r < R; ++r) {
+                        double s = a[r];
+                        for (uint32_t k = 0; k < r; ++k) s -= L[r * R + k] * x[k];
+                        x[r] = s / L[r * R + r];
+                    }
+                    for (uint32_t r = R; r-- > 0;) {
+                        double s = x[r];
+                        for (uint32_t k = r + 1; k < R; ++k) s -= L[k * R + r] * x[k];
+                        x[r] = s / L[r * R + r];
+                    }
+                } else {
+                    for (uint32_t r = 0; r < R; ++r) {
+                        double s = a[piv[r]];
+                        for (uint32_t k = 0; k < r; ++k) s -= L[r * R + k] * x[k];
+                        x[r] = s;
+                    }
+                    for (uint32_t r = R; r-- > 0;) {
+                        double s = x[r];
+                        for (uint32_t k = r + 1; k < R; ++k) s -= L[r * R + k] * x[k];
+                        x[r] = s / L[r * R + r];
+                    }
+                }
+                for (uint32_t r = 0; r < R; ++r) a[r] = (float)x[r];
+            }
+            free(x);
+        }
+    }
+    free(piv);
+    free(L);
+    return status;
+}
+
+/* Column norms of A into lambda and division of the columns by them
+ * (src/matrix/matrix.c:389-469: 2-norm; :472-553: the column maximum, not
+ * the absolute value, floored at 0 and clamped to >= 1). */
+static void orc_normalize(float *A, uint64_t I, uint32_t R, uint32_t stride, int maxnorm, float *lambda, int nthreads) {
+    double *part = (double *)calloc((size_t)nthreads * R, sizeof(double));
+#pragma omp parallel num_threads(nthreads)
+    {
+#ifdef _OPENMP
+        double *my = part + (size_t)omp_get_thread_num() * R;
+#else
+        double *my = part;
+#endif
+#pragma omp for schedule(static)
+        for (uint64_t i = 0; i < I; ++i)
+            for (uint32_t r = 0; r < R; ++r) {
+                const double v = A[i * stride + r];
+                if (maxnorm)
+                    my[r] = v > my[r] ? v : my[r];
+                else
+                    my[r] += v * v;
+            }
+    }
+    for (uint32_t r = 0; r < R; ++r) {
+        double t = 0.0;
+        for (int th = 0; th < nthreads; ++th) {
+            const double v = part[(size_t)th * R + r];
+            t = maxnorm ? (v > t ? v : t) : t + v;
+        }
+        lambda[r] = maxnorm ? (float)(t < 1.0 ? 1.0 : t) : (float)sqrt(t);
+    }
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (uint64_t i = 0; i < I; ++i)
+        for (uint32_t r = 0; r < R; ++r) A[i * stride + r] /= lambda[r];
+    free(part);
+}
+
+/* CP-ALS of src/sptensor/cpd_omp.c:32-244 (OmpCpdAlsStep, use_reduce = 0)
+ * from the caller's factors facs[m] (ndims[m] x stride FP32, updated in
+ * place): per mode the OpenMP COO MTTKRP (orc_omp_mttkrp_coo, mats_order
+ * m, m+1, ...), the normal-equation solve against the Hadamard product of
+ * the other modes' FP64 Grams (orc_solve_rows), the column norms (2-norm on
+ * the first iteration, then the clamped max-norm) and the Gram refresh; then
+ * the fit of src/ktensor/ktensor.c:77-240 with the full Grams (the
+ * reference reads a triangle it never writes, ktensor.c:128-143); stop after
+ * niters or when the fit changes by less than tol; finally GetFinalLambda
+ * (matrix.c:556-572).  fit_trace (may be NULL) gets one fit per iteration;
+ * t_mttkrp / t_other (may be NULL) the seconds spent in the MTTKRPs and in
+ * everything else.  Returns the last fit. */
+double orc_omp_cpd_als(uint32_t N, uint64_t nnz, const uint32_t *const *inds, const float *vals,
+                       const uint32_t *ndims, uint32_t R, uint32_t stride, uint32_t niters, double tol,
+                       float *const *facs, float *lambda, double *fit_trace, uint32_t *iters_done, double *t_mttkrp,
+                       double *t_other, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+    nthreads = 1;
+#endif
+    const double t_start = orc_wtime();
+    double tm = 0.0;
+    uint64_t max_dim = 0;
+    for (uint32_t m = 0; m < N; ++m) max_dim = ndims[m] > max_dim ? ndims[m] : max_dim;
+    float *tmp = (float *)calloc(max_dim * stride + 1, sizeof(float));
+    double *ata = (double *)calloc((size_t)(N + 1) * R * R, sizeof(double));
+    double *V = (double *)malloc((size_t)R * R * sizeof(double));
+    for (uint32_t m = 0; m < N; ++m) orc_gram(facs[m], ndims[m], R, stride, ata + (size_t)m * R * R, nthreads);
+    double normX2 = 0.0;
+#pragma omp parallel for reduction(+ : normX2) schedule(static) num_threads(nthreads)
+    for (uint64_t z = 0; z < nnz; ++z) normX2 += (double)vals[z] * vals[z];
+    uint32_t order[8];
+    double fit = 0.0, oldfit = 0.0;
+    uint32_t it = 0;
+    while (it < niters) {
+        for (uint32_t m = 0; m < N; ++m) {
+            order[0] = m;
+            for (uint32_t i = 1; i < N; ++i) order[i] = (m + i) % N;
+            const double t0 = orc_wtime();
+            orc_omp_mttkrp_coo(N, nnz, inds, vals, R, stride, (const float *const *)facs, order, m, ndims[m], tmp,
+                               nthreads);
+            tm += orc_wtime() - t0;
+            memcpy(facs[m], tmp, (size_t)ndims[m] * stride * sizeof(float));
+            for (uint32_t p = 0; p < R * R; ++p) V[p] = 1.0;
+            for (uint32_t k = 0; k < N; ++k)
+                if (k != m)
+                    for (uint32_t p = 0; p < R * R; ++p) V[p] *= ata[(size_t)k * R * R + p];
+            orc_solve_rows(V, R, facs[m], ndims[m], stride, nthreads);
+            orc_normalize(facs[m], ndims[m], R, stride, it > 0, lambda, nthreads);
+            orc_gram(facs[m], ndims[m], R, stride, ata + (size_t)m * R * R, nthreads);
+        }
+        /* ||K||^2 = lambda^T (Hadamard of all Grams) lambda; <X,K> from the
+         * last mode's factor and its MTTKRP output (still in tmp). */
+        double normK = 0.0;
+        for (uint32_t p = 0; p < R; ++p)
+            for (uint32_t q = 0; q < R; ++q) {
+                double h = 1.0;
+                for (uint32_t k = 0; k < N; ++k) h *= ata[(size_t)k * R * R + p * R + q];
+                normK += (double)lambda[p] * lambda[q] * h;
+            }
+        normK = fabs(normK);
+        double inner = 0.0;
+        const float *last = facs[N - 1];
+        for (uint32_t r = 0; r < R; ++r) {
+            double col = 0.0;
+#pragma omp parallel for reduction(+ : col) schedule(static) num_threads(nthreads)
+            for (uint64_t i = 0; i < ndims[N - 1]; ++i) col += (double)last[i * stride + r] * tmp[i * stride + r];
+            inner += lambda[r] * col;
+        }
+        double residual = normX2 + normK - 2 * inner;
+        if (residual > 0) residual = sqrt(residual);
+        fit = 1 - residual / sqrt(normX2);
+        if (fit_trace) fit_trace[it] = fit;
+        ++it;
+        if (it > 1 && fabs(fit - oldfit) < tol) break;
+        oldfit = fit;
+    }
+    /* GetFinalLambda: every factor to unit 2-norm columns, lambda *= norms. */
+    float *nrm = (float *)malloc((R ? R : 1) * sizeof(float));
+    for (uint32_t m = 0; m < N; ++m) {
+        orc_normalize(facs[m], ndims[m], R, stride, 0, nrm, nthreads);
+        for (uint32_t r = 0; r < R; ++r) lambda[r] *= nrm[r];
+    }
+    free(nrm);
+    free(V);
+    free(ata);
+    free(tmp);
+    if (iters_done) *iters_done = it;
+    if (t_mttkrp) *t_mttkrp = tm;
+    if (t_other) *t_other = orc_wtime() - t_start - tm;
+    return fit;
 }
 
 /* src/sptensor/hicoo/mttkrp.c:232-310 (3-D blocked) and :74-151 (N-D). */

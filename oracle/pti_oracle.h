@@ -23,6 +23,20 @@ void orc_mttkrp_f64(uint32_t N, uint64_t nnz, const uint32_t *const *inds, const
                     uint32_t R, uint32_t stride, const float *const *facs, const uint32_t *order,
                     uint32_t mode, uint64_t nrows_out, double *out);
 
+/* orc_mttkrp_f64 restricted to the output rows sel[0..nsel-1]: out is
+ * nsel x R doubles (row s = output row sel[s]); OpenMP with nthreads threads
+ * (<= 0: all).  For full-size accuracy checks. */
+void orc_mttkrp_f64_rows(uint32_t N, uint64_t nnz, const uint32_t *const *inds, const float *vals, uint32_t R,
+                         uint32_t stride, const float *const *facs, uint32_t mode, uint64_t nrows_out,
+                         uint32_t nsel, const uint32_t *sel, double *out, int nthreads);
+
+/* CP-completion in FP64: *loss = 0.5 sum_z e_z^2 with
+ * e_z = v_z - sum_r prod_m A_m[i_m][r], and out (nsel x R) the MTTKRP of the
+ * residuals at output rows sel[] of `mode` (-dL/dA_mode there). */
+void orc_completion_f64(uint32_t N, uint64_t nnz, const uint32_t *const *inds, const float *vals, uint32_t R,
+                        uint32_t stride, const float *const *facs, uint32_t mode, uint64_t nrows_out,
+                        uint32_t nsel, const uint32_t *sel, double *out, double *loss, int nthreads);
+
 /* Sequential FP32 MTTKRP in the reference's operation order
  * (src/sptensor/mttkrp.c:39-163): out zeroed over nrows_out*stride floats. */
 void orc_mttkrp_coo_seq(uint32_t N, uint64_t nnz, const uint32_t *const *inds, const float *vals,
@@ -33,6 +47,16 @@ void orc_mttkrp_coo_seq(uint32_t N, uint64_t nnz, const uint32_t *const *inds, c
 void orc_omp_mttkrp_coo(uint32_t N, uint64_t nnz, const uint32_t *const *inds, const float *vals,
                         uint32_t R, uint32_t stride, const float *const *facs, const uint32_t *order,
                         uint32_t mode, uint64_t nrows_out, float *out, int nthreads);
+
+/* CP-ALS on the host: src/sptensor/cpd_omp.c:32-244 (OmpCpdAlsStep with
+ * use_reduce = 0) with the OpenMP COO MTTKRP above and an FP64
+ * Cholesky-Banachiewicz / Doolittle-LU normal-equation solve.  facs (ndims[m]
+ * x stride FP32) are the initial factors and receive the final ones; lambda
+ * gets R values.  Returns the last fit. */
+double orc_omp_cpd_als(uint32_t N, uint64_t nnz, const uint32_t *const *inds, const float *vals,
+                       const uint32_t *ndims, uint32_t R, uint32_t stride, uint32_t niters, double tol,
+                       float *const *facs, float *lambda, double *fit_trace, uint32_t *iters_done, double *t_mttkrp,
+                       double *t_other, int nthreads);
 
 /* Sequential HiCOO MTTKRP (src/sptensor/hicoo/mttkrp.c:61-152, 232-310).
  * binds[m][b], einds[m][z]; kptr has nk+1 block offsets. */

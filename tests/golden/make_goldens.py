@@ -18,7 +18,11 @@ equal keys) and a 3-D tensor with indices >= 2^24 (the Morton key ignores
 bits 24-31).  Inputs are stored in the fixtures, so the tests do not depend on
 the generator.
 
-    make -f oracle/ref.mk && python tests/golden/make_goldens.py
+Also writes dense_helpers.npz (dense_helpers below): the reference's column
+norms, GetFinalLambda and the text dumps of a matrix, a Kruskal tensor and a
+HiCOO tensor.
+
+    make -f oracle/ref.mk && python tests/golden/make_goldens.py [--dense-only]
 """
 import ctypes as C
 import hashlib
@@ -166,7 +170,113 @@ def digest(arrs):
     return h.hexdigest()
 
 
+class _CFile:
+    """libc FILE* for the reference's dump functions."""
+    libc = C.CDLL(None)
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        self.fp = self.libc.fopen(self.path.encode(), b"w")
+        assert self.fp
+        return C.c_void_p(self.fp)
+
+    def __exit__(self, *exc):
+        self.libc.fclose(C.c_void_p(self.fp))
+
+
+def ref_dump(fn, obj, path):
+    with _CFile(path) as fp:
+        rc = fn(C.byref(obj), fp)
+    assert rc == 0
+    with open(path, "rb") as f:
+        return np.frombuffer(f.read(), np.uint8).copy()
+
+
+def dense_helpers():
+    """The reference's column norms, final lambda and text dumps
+    (src/matrix/matrix.c:389-572, src/matrix/dump.c:33-49,
+    src/ktensor/dump.c:23-51, src/sptensor/hicoo/dump.c:30-74) on seeded
+    inputs.  Column 3 of the matrix is all negative and column 5 lies in
+    (0, 0.5): the max-norm takes the column maximum (not the absolute value),
+    floors it at 0 and clamps it to >= 1, which these columns exercise."""
+    import tempfile
+    ref.ptiMatrix2Norm.argtypes = [C.POINTER(F.ptiMatrix), C.POINTER(F.ptiValue)]
+    ref.ptiMatrixMaxNorm.argtypes = [C.POINTER(F.ptiMatrix), C.POINTER(F.ptiValue)]
+    ref.GetFinalLambda.argtypes = [F.ptiIndex, F.ptiIndex, C.POINTER(C.POINTER(F.ptiMatrix)),
+                                   C.POINTER(F.ptiValue)]
+    ref.GetFinalLambda.restype = None
+    ref.ptiDumpMatrix.argtypes = [C.POINTER(F.ptiMatrix), C.c_void_p]
+    ref.ptiDumpKruskalTensor.argtypes = [C.POINTER(F.ptiKruskalTensor), C.c_void_p]
+    ref.ptiDumpSparseTensorHiCOO.argtypes = [C.POINTER(F.ptiSparseTensorHiCOO), C.c_void_p]
+    rng = np.random.default_rng(2024)
+    rec = {}
+    A = rng.uniform(-3, 3, (37, 10)).astype(np.float32)
+    A[:, 3] = -rng.uniform(0.5, 2, 37)
+    A[:, 5] = rng.uniform(0, 0.5, 37)
+    rec["A"] = A
+    for name, fn in (("2norm", ref.ptiMatrix2Norm), ("maxnorm", ref.ptiMatrixMaxNorm)):
+        m, full = ref_matrix(A)
+        lam = (F.ptiValue * 10)()
+        assert fn(C.byref(m), lam) == 0
+        rec[name + "_lambda"] = np.array(lam[:10], np.float32)
+        rec[name + "_A"] = full[:37, :10].copy()
+        ref.ptiFreeMatrix(C.byref(m))
+    # GetFinalLambda over three factors, lambda starting from given values.
+    facs = [rng.uniform(-1, 2, (d, 6)).astype(np.float32) for d in (20, 15, 9)]
+    lam0 = rng.uniform(0.5, 3, 6).astype(np.float32)
+    mats = [ref_matrix(f) for f in facs]
+    arr = (C.POINTER(F.ptiMatrix) * 3)(*[C.pointer(m) for m, _ in mats])
+    lam = (F.ptiValue * 6)(*lam0)
+    ref.GetFinalLambda(6, 3, arr, lam)
+    for i, f in enumerate(facs):
+        rec["final_in%d" % i] = f
+        rec["final_out%d" % i] = mats[i][1][:f.shape[0], :6].copy()
+    rec["final_lambda_in"] = lam0
+    rec["final_lambda_out"] = np.array(lam[:6], np.float32)
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "dump.txt")
+        m, _ = ref_matrix(A)
+        rec["dump_matrix"] = ref_dump(ref.ptiDumpMatrix, m, path)
+        ref.ptiFreeMatrix(C.byref(m))
+        k = F.ptiKruskalTensor()
+        k.nmodes = 3
+        k.rank = 6
+        nd = (F.ptiIndex * 3)(20, 15, 9)
+        k.ndims = nd
+        klam = (F.ptiValue * 6)(*rec["final_lambda_out"])
+        k.lambda_ = klam
+        k.fit = 0.123456789
+        k.factors = arr
+        rec["dump_ktensor"] = ref_dump(ref.ptiDumpKruskalTensor, k, path)
+        nd3, i3, v3 = load_tns("3d-24.tns")
+        t = ref_tensor(nd3, i3, v3)
+        h = F.ptiSparseTensorHiCOO()
+        mx = F.ptiNnzIndex()
+        assert ref.ptiSparseTensorToHiCOO(C.byref(h), C.byref(mx), C.byref(t), 1, 1, 2, 1) == 0
+        rec["hicoo_params"] = np.array([1, 1, 2], np.uint32)
+        rec["hicoo_ndims"] = np.array(nd3, np.uint32)
+        rec["hicoo_in_inds"] = i3
+        rec["hicoo_in_vals"] = v3
+        rec["dump_hicoo"] = ref_dump(ref.ptiDumpSparseTensorHiCOO, h, path)
+        ref.ptiFreeSparseTensorHiCOO(C.byref(h))
+        ref.ptiFreeSparseTensor(C.byref(t))
+    for mm, _ in mats:
+        ref.ptiFreeMatrix(C.byref(mm))
+    path = os.path.join(OUT, "dense_helpers.npz")
+    np.savez_compressed(path, **rec)
+    print("wrote", path, flush=True)
+
+
 def main():
+    if "--dense-only" in sys.argv:
+        dense_helpers()
+        return
+    dense_helpers()
     P_SMALL = [(1, 2, 2), (2, 3, 4)]
     for fname in ("3d_7.tns", "3d-24.tns", "3d_dense.tns"):
         nd, i, v = load_tns(fname)

@@ -145,6 +145,16 @@ def test_mttkrp_shape_errors_are_reported_before_device_work():
     assert e.value.code == F.PTIERR_SHAPE_MISMATCH
 
 
+def test_mttkrp_order_must_be_a_permutation_of_the_other_modes():
+    t = COOTensor.from_arrays([4, 5, 6], np.array([[0], [1], [2]], np.uint32), np.ones(1, np.float32))
+    h = t.toHiCOO(1, 1, 1)
+    for order in ([0, 1, 1], [0, 2, 2], [0, 0, 1], [0, 1, 3]):
+        for obj in (t, h):
+            with pytest.raises(PtiError) as e:
+                obj.MTTKRP(0, _mats([4, 5, 6], 8), order)
+            assert e.value.code == F.PTIERR_VALUE_ERROR, order
+
+
 def test_mttkrp_without_gpu_fails_loudly(gpu_available):
     if gpu_available:
         pytest.skip("a GPU is present")

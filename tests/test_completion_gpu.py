@@ -81,6 +81,9 @@ SHAPES = [
     ((9, 8, 7, 6, 5), 4000, 33),    # order 5
     ((200, 150, 100), 70000, 64),   # many chunks, ragged last chunk
     ((12, 11, 10), 500, 256),       # largest rank the forward kernel takes
+    ((60, 50, 40), 5000, 65),       # odd ranks above 64: lane groups step over column blocks
+    ((60, 50, 40), 5000, 75),
+    ((60, 50, 40), 5000, 130),      # 2 mod 4 above 128
 ]
 
 

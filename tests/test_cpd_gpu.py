@@ -199,6 +199,99 @@ def test_bf16_then_fp32_schedule_reaches_the_fp32_fit():
     assert rmix.fits[:10] != r32.fits[:10]
 
 
+@pytest.mark.parametrize("R", [65, 75, 130])
+def test_cp_als_coo_wide_odd_ranks_follow_fp64_oracle(R):
+    """Ranks above 64 that are odd or 2 mod 4 run the COO kernel in column
+    tiles of 64 lanes x 1 or 2 ranks.  Gaussian initial factors keep the
+    Hadamard of the Grams well conditioned, so the FP32 device run and the
+    FP64 oracle stay close over the iterations."""
+    ndims = (400, 300, 250)
+    inds, vals = random_sparse(ndims, 30000, R)
+    dev, _keep = device_tensor("coo", list(ndims), inds, vals)
+    rng = np.random.default_rng(R)
+    init = [rng.standard_normal((d, R)).astype(np.float32) for d in ndims]
+    res = D.cp_als(dev, R, niters=3, tol=0.0, factors=[cuda(f) for f in init])
+    lam, fs, fits = A.cp_als(inds, vals, [f.astype(np.float64) for f in init], 3, 0.0)
+    assert np.allclose(res.fits, fits, atol=1e-4, rtol=0), (res.fits, fits)
+    assert rel_fro(res.lambda_.cpu().numpy(), lam) <= 1e-3
+    for f, g in zip(res.factors, fs):
+        assert rel_fro(f.cpu().numpy(), g) <= 1e-3
+
+
+def test_dropin_cpd_als_runs_at_odd_rank_above_64():
+    # The reference's i+j+1 start makes the normal equations singular at this
+    # rank, so only the call's completion and finite results are checked here;
+    # the COO MTTKRP it runs is checked against the oracle above.
+    ndims = [200, 150, 100]
+    inds, vals = random_sparse(ndims, 5000, 21)
+    t = COOTensor.from_arrays(ndims, inds, vals)
+    k = F.ptiKruskalTensor()
+    assert lib().ptiNewKruskalTensor(C.byref(k), 3, (F.ptiIndex * 3)(*ndims), 65) == 0
+    assert lib().ptiCpdAls(C.byref(t.address), 65, 2, 0.0, C.byref(k)) == 0
+    assert np.isfinite(k.fit)
+    assert np.all(np.isfinite(np.array(k.lambda_[:65])))
+    lib().ptiFreeKruskalTensor(C.byref(k))
+
+
+C4_DIMS = [1000000, 500000, 100000, 1000, 24]
+
+
+def c4_tensor(nnz, seed):
+    """The end-to-end CP-ALS configuration's shape and index distributions
+    (order 5, Zipf(1.1) on modes 0-2, uniform on modes 3-4, U(0,1) values) at
+    a smaller nonzero count."""
+    return COOTensor.synthetic(C4_DIMS, nnz, dist=[1, 1, 1, 0, 0], zipf_s=[1.1, 1.1, 1.1, 1, 1],
+                               permute=[0] * 5, seed=seed)
+
+
+@pytest.mark.parametrize("bf16_iters", [0, 3])
+def test_cp_als_order5_c4_shape_follows_fp64_oracle(bf16_iters, oracle):
+    """Device CP-ALS (HiCOO, stream kernel of order 5, R = 16) on a 1.5M-nonzero
+    tensor of the CP-ALS configuration's shape against the oracle's FP64 ALS
+    from the same factors over 6 iterations, all-FP32 and with the first 3
+    iterations' MTTKRPs on BF16 factor copies (the oracle rounds its factor
+    copies the same way)."""
+    R = 16
+    X = c4_tensor(1_500_000, 44)
+    inds, vals = X.inds(), X.values()
+    H = X.toHiCOO(7, 14, 14)
+    dev = DevHiCOO(H)
+    init = D.init_factors(C4_DIMS, R, "uniform", seed=4)
+    init_np = [f.cpu().numpy().astype(np.float64) for f in init]
+    res = D.cp_als(dev, R, niters=6, tol=0.0, factors=[f.clone() for f in init], bf16_iters=bf16_iters)
+
+    def mttkrp(i, v, f, m):
+        return oracle.mttkrp(i, v, f, m, kind="f64")
+    lam, fs, fits = A.cp_als(inds, vals, init_np, 6, 0.0, bf16_iters=bf16_iters, mttkrp=mttkrp)
+    print("order-5 CP-ALS, bf16_iters=%d: device fits %s, oracle fits %s" % (bf16_iters, res.fits, fits), flush=True)
+    assert res.iters == 6
+    assert np.allclose(res.fits, fits, atol=1e-4, rtol=0), (res.fits, fits)
+    assert rel_fro(res.lambda_.cpu().numpy(), lam) <= 1e-3
+    for f, g in zip(res.factors, fs):
+        assert rel_fro(f.cpu().numpy(), g) <= 1e-3
+
+
+def test_bf16_then_fp32_reaches_the_fp32_fit_on_a_planted_order5_tensor():
+    """Every entry of a 12 x 10 x 9 x 8 x 6 tensor observed, values from a
+    planted rank-3 model: all-FP32 ALS and the schedule with 10 BF16
+    iterations first must end at the same fit (within 1e-4) after 60
+    iterations, both near 1."""
+    shape = (12, 10, 9, 8, 6)
+    rng = np.random.default_rng(8)
+    P = [rng.standard_normal((d, 3)) for d in shape]
+    inds = np.array(np.unravel_index(np.arange(int(np.prod(shape))), shape), np.uint32)
+    prod = np.ones((inds.shape[1], 3))
+    for m, p in enumerate(P):
+        prod *= p[inds[m]]
+    vals = prod.sum(axis=1).astype(np.float32)
+    dev, _keep = device_tensor("hicoo", list(shape), inds, vals)
+    r32 = D.cp_als(dev, 3, niters=60, tol=0.0, seed=2)
+    rmix = D.cp_als(dev, 3, niters=60, tol=0.0, seed=2, bf16_iters=10)
+    print("planted order 5: FP32 fit %.8f, BF16-then-FP32 fit %.8f" % (r32.fit, rmix.fit), flush=True)
+    assert r32.fit > 0.999
+    assert abs(rmix.fit - r32.fit) <= 1e-4, (rmix.fit, r32.fit)
+
+
 def test_dropin_cpd_als_with_reference_init():
     # Rank 1: the reference's initial factor (i + 1) is well posed.
     ndims = [20, 15, 10]

@@ -178,3 +178,46 @@ def test_rank_matrix_and_rank_ktensor_containers():
     assert k.nmodes == 3 and k.rank == 7 and list(k.ndims[:3]) == [4, 5, 6]
     lib().ptiFreeRankKruskalTensor(C.byref(k))
     assert k.nmodes == 0
+
+
+def test_oracle_bf16_rounding_is_round_to_nearest_even():
+    import torch
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.standard_normal(20000) * 10 ** rng.uniform(-20, 20, 20000),
+                        # ties: the low 16 bits exactly 0x8000, even and odd upper halves
+                        np.array([0x3F808000, 0x3F818000, 0xBF808000], np.uint32).view(np.float32)]).astype(np.float32)
+    ref = torch.from_numpy(x).to(torch.bfloat16).float().numpy()
+    assert np.array_equal(A.to_bf16(x).astype(np.float32), ref)
+    assert A.to_bf16(np.array([0x3F808000], np.uint32).view(np.float32))[0] == 1.0
+
+
+def test_host_omp_cpd_als_follows_the_fp64_oracle(oracle):
+    """orc_omp_cpd_als, the CPU baseline's restatement of ptiOmpCpdAls
+    (src/sptensor/cpd_omp.c:32-244), follows the FP64 ALS oracle from the same
+    factors: fits, lambda and factors to FP32 rounding, at 1 and 4 threads."""
+    rng = np.random.default_rng(7)
+    for ndims, R in (((30, 25, 20), 8), ((12, 10, 9, 8, 6), 5)):
+        nnz = 3000
+        flat = rng.choice(int(np.prod(ndims)), size=nnz, replace=False)
+        inds = np.array(np.unravel_index(np.sort(flat), ndims), np.uint32)
+        vals = rng.uniform(0, 1, nnz).astype(np.float32)
+        init = [rng.uniform(0, 1, (d, R)).astype(np.float32) for d in ndims]
+        lam_o, fs_o, fits_o = A.cp_als(inds, vals, [f.astype(np.float64) for f in init], 5, 0.0)
+        for nt in (1, 4):
+            lam, fs, fits, tm, to = oracle.omp_cpd_als(inds, vals, ndims, init, 5, 0.0, nthreads=nt)
+            assert np.allclose(fits, fits_o, atol=1e-5, rtol=0), (fits, fits_o)
+            assert rel_fro(lam, lam_o) <= 1e-4
+            for f, g in zip(fs, fs_o):
+                assert rel_fro(f, g) <= 1e-4
+            assert tm >= 0 and to >= 0
+
+
+def test_host_omp_cpd_als_recovers_exact_low_rank(oracle):
+    shape = (20, 15, 10)
+    rng = np.random.default_rng(2)
+    P = [rng.standard_normal((d, 3)) for d in shape]
+    vals = np.einsum("ir,jr,kr->ijk", *P).ravel().astype(np.float32)
+    inds = np.array(np.unravel_index(np.arange(vals.size), shape), np.uint32)
+    init = [rng.uniform(0, 1, (d, 3)).astype(np.float32) for d in shape]
+    lam, fs, fits, _, _ = oracle.omp_cpd_als(inds, vals, shape, init, 300, 1e-10, nthreads=2)
+    assert fits[-1] > 0.999, fits[-5:]

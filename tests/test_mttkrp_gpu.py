@@ -4,8 +4,9 @@ Every result is compared with the FP64 oracle (relative Frobenius error
 <= 1e-5, the FP32 tolerance) and, where the reference's own FP32 output is in
 the fixtures, with that output (<= 1e-5).  Covers the drop-in entry points
 ptiMTTKRP / ptiMTTKRPHiCOO on the reference fixtures, orders 2-8, ranks
-1-300 (including ranks that are not multiples of 4 and ranks above one
-256-rank column tile), non-default mats_order, empty and single-nonzero
+1-300 (including odd ranks and ranks 2 mod 4 above 64, which COO cuts into
+column tiles of 64 lanes x 1 or 2 ranks, and ranks above one 256-rank column
+tile), non-default mats_order, empty and single-nonzero
 tensors, registered (device-resident) tensors, the device API on torch
 tensors, and the BF16-factor form against the FP64 oracle evaluated on the
 same bf16-rounded factors.
@@ -90,6 +91,15 @@ SHAPES = [
     ([120, 90, 70], 3000, 100, 3, 5),
     ([120, 90, 70], 3000, 80, 3, 5),
     ([120, 90, 70], 3000, 300, 3, 5),
+    # Ranks above 64 whose parity leaves one or two ranks per lane (COO column
+    # tiles of 64 x V ranks), and a last HiCOO rank tile narrower than the
+    # first tile's vector width.
+    ([120, 90, 70], 3000, 65, 3, 5),
+    ([120, 90, 70], 3000, 66, 3, 5),
+    ([120, 90, 70], 3000, 75, 3, 5),
+    ([120, 90, 70], 3000, 127, 3, 5),
+    ([120, 90, 70], 3000, 130, 3, 5),
+    ([120, 90, 70], 3000, 255, 3, 5),
     ([40, 30, 20, 10], 4000, 16, 2, 3),
     ([30, 20, 15, 10, 8], 5000, 16, 1, 3),
     ([12, 11, 10, 9, 8, 7], 4000, 8, 1, 2),
@@ -222,6 +232,38 @@ def test_stream_kernel_zipf_ranks_and_orders(ndims, R, oracle):
             out = dh.mttkrp_bf16(bf, mode)
             torch.cuda.synchronize()
             assert rel_fro(out.cpu().numpy()[:, :R], oracle.mttkrp(inds, vals, rounded, mode, kind="f64")) <= TOL, mode
+
+
+@pytest.mark.parametrize("ndims,R", [([3000, 2500, 2000], 32), ([3000, 2500, 2000], 40), ([3000, 2500, 2000], 64),
+                                     ([3000, 2500, 2000], 96), ([3000, 2500, 2000], 128), ([400, 300, 200, 50], 64),
+                                     ([300, 200, 100, 40, 20], 32), ([4000, 3000], 48)])
+def test_bf16_eight_ranks_per_lane(ndims, R, oracle):
+    """BF16 factors at ranks that are multiples of 8 and at least 32 gather
+    eight ranks (16 bytes) per lane on the stream kernel, in lane groups of 4
+    or 8 lanes with a stream form of their own; ranks above 64 cut into column
+    tiles that share the first tile's lane groups.  Against the FP64 oracle on
+    the same bf16-rounded factors, every mode, with FP32 calls interleaved (the
+    FP32 and BF16 stream forms of a mode are cached side by side)."""
+    import torch
+    from hiparti_amd.device import DevHiCOO
+    N = len(ndims)
+    t = COOTensor.synthetic(ndims, 200000, dist=[1] * N, zipf_s=[1.1] * N, permute=[1] * N, seed=R + N)
+    inds, vals = t.inds(), t.values()
+    rng = np.random.default_rng(R)
+    facs = [rng.random((d, R)).astype(np.float32) for d in ndims]
+    dfacs = [torch.from_numpy(f).cuda() for f in facs]
+    dh = DevHiCOO(t.toHiCOO(7, 14, 14))
+    bf = [f.to(torch.bfloat16) for f in dfacs]
+    rounded = [b.float().cpu().numpy() for b in bf]
+    for rep in range(2):
+        for mode in range(N):
+            out = dh.mttkrp_bf16(bf, mode)
+            torch.cuda.synchronize()
+            ref = oracle.mttkrp(inds, vals, rounded, mode, kind="f64")
+            assert rel_fro(out.cpu().numpy()[:, :R], ref) <= TOL, (rep, mode)
+            out = dh.mttkrp(dfacs, mode)
+            torch.cuda.synchronize()
+            assert rel_fro(out.cpu().numpy()[:, :R], oracle.mttkrp(inds, vals, facs, mode, kind="f64")) <= TOL
 
 
 def test_large_zipf_properties(oracle):
