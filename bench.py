@@ -45,7 +45,6 @@ Other legs (each skippable):
                     [--no-precision] [--no-completion]
 """
 import argparse
-import ctypes as C
 import json
 import os
 import re
@@ -202,7 +201,11 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libhiparti_ref.so")
+# The oracle's OpenMP restatements against the reference's own ptiOmpMTTKRP /
+# ptiOmpMTTKRPHiCOO, both built from source and timed on the same 2M-nonzero
+# sample of the 100M-nonzero tensor on 8 host cores (ms per 3-mode step,
+# extrapolated): restatement / reference.
+PORT_VS_REFERENCE = {"hicoo": 1.03, "coo": 1.02}
 
 
 def _sample_prefix(h, sample_nnz):
@@ -233,54 +236,30 @@ def _time_modes(run, N):
 def cpu_baseline(h, ndims, facs_np, R, sample_nnz, nthreads):
     """The CPU baseline on the leading superblocks holding about sample_nnz
     nonzeros of the same HiCOO tensor, one warm-up and two timed calls per
-    mode, extrapolated per nonzero to the whole tensor (ms per three-mode
-    step).  It is the reference's own ptiOmpMTTKRPHiCOO with tk = host
-    threads and tb = 1 (superblock-parallel with per-element atomics,
-    src/sptensor/hicoo/mttkrp_omp.c:184-204,400-467) when its library has been
-    built from the reference sources (oracle/_ref), else the oracle's
-    restatement of that kernel.  Returns (ms, kind, z_end, k_end)."""
-    from hiparti_amd import _ffi as F
+    mode, extrapolated per nonzero to the whole tensor (ms per N-mode step).
+    It is the oracle's restatement of the reference's ptiOmpMTTKRPHiCOO with
+    tk = host threads and tb = 1 (superblock-parallel with per-element
+    atomics, src/sptensor/hicoo/mttkrp_omp.c:184-204,400-467); the reference
+    library itself is built only where its sources are, to pin that
+    restatement (PORT_VS_REFERENCE).  Returns (ms, kind, z_end, k_end)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
     a = h.address
     N = len(ndims)
     k_end, b_end, z_end = _sample_prefix(h, sample_nnz)
-    if os.path.exists(REF_LIB):
-        L = C.CDLL(REF_LIB)
-        L.ptiOmpMTTKRPHiCOO.argtypes = [C.POINTER(F.ptiSparseTensorHiCOO), C.POINTER(C.POINTER(F.ptiMatrix)),
-                                        C.POINTER(F.ptiIndex), F.ptiIndex, C.c_int, C.c_int]
-        L.ptiOmpMTTKRPHiCOO.restype = C.c_int
-        view = F.ptiSparseTensorHiCOO.from_buffer_copy(a)   # same arrays, first k_end superblocks
-        view.kptr.len = k_end + 1
-        view.nnz = z_end
-        mats = []
-        for f in facs_np:
-            m = F.ptiMatrix(f.shape[0], f.shape[1], f.shape[0], f.shape[1], f.ctypes.data_as(C.POINTER(F.ptiValue)))
-            mats.append(m)
-        out = np.zeros((max(ndims), R), np.float32)
-        mats.append(F.ptiMatrix(max(ndims), R, max(ndims), R, out.ctypes.data_as(C.POINTER(F.ptiValue))))
-        ptrs = (C.POINTER(F.ptiMatrix) * (N + 1))(*[C.pointer(m) for m in mats])
+    kptr = np.ctypeslib.as_array(a.kptr.data, shape=(int(a.kptr.len),))
+    sub = {
+        "kptr": kptr[:k_end + 1].copy(),
+        "bptr": np.ctypeslib.as_array(a.bptr.data, shape=(b_end + 1,)).copy(),
+        "binds": np.stack([np.ctypeslib.as_array(a.binds[m].data, shape=(b_end,)) for m in range(N)]),
+        "einds": np.stack([np.ctypeslib.as_array(a.einds[m].data, shape=(z_end,)) for m in range(N)]),
+        "values": np.ctypeslib.as_array(a.values.data, shape=(z_end,)).copy(),
+    }
 
-        def run(mode):
-            order = (F.ptiIndex * N)(*([mode] + [(mode + i) % N for i in range(1, N)]))
-            mats[N].nrows = ndims[mode]
-            assert L.ptiOmpMTTKRPHiCOO(C.byref(view), ptrs, order, mode, int(nthreads), 1) == 0
-        kind = "reference"
-    else:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        kptr = np.ctypeslib.as_array(a.kptr.data, shape=(int(a.kptr.len),))
-        sub = {
-            "kptr": kptr[:k_end + 1].copy(),
-            "bptr": np.ctypeslib.as_array(a.bptr.data, shape=(b_end + 1,)).copy(),
-            "binds": np.stack([np.ctypeslib.as_array(a.binds[m].data, shape=(b_end,)) for m in range(N)]),
-            "einds": np.stack([np.ctypeslib.as_array(a.einds[m].data, shape=(z_end,)) for m in range(N)]),
-            "values": np.ctypeslib.as_array(a.values.data, shape=(z_end,)).copy(),
-        }
-
-        def run(mode):
-            O.mttkrp_hicoo(sub, int(a.sb_bits), facs_np, mode, kind="omp", nthreads=nthreads)
-        kind = "port"
+    def run(mode):
+        O.mttkrp_hicoo(sub, int(a.sb_bits), facs_np, mode, kind="omp", nthreads=nthreads)
     total = _time_modes(run, N)
-    return total / z_end * int(a.nnz) * 1e3, kind, z_end, k_end
+    return total / z_end * int(a.nnz) * 1e3, "port", z_end, k_end
 
 
 def coo_leg(X, ndims, dfacs, facs_np, outs, R, args, nthreads, rows=None):
@@ -289,12 +268,10 @@ def coo_leg(X, ndims, dfacs, facs_np, outs, R, args, nthreads, rows=None):
     tensor and factors: ms per three-mode step with HIP events,
     its roofline with the reference's COO convention B = nnz*(N*4 + 4) +
     sum_m ndims[m]*R*4 (benchmark/tensor/mttkrp.c:304-307), and the CPU
-    baseline: the reference's own ptiOmpMTTKRP (src/sptensor/mttkrp_omp.c:
-    125-205, nonzero-parallel with atomics) on the first cpu_sample_nnz
-    nonzeros, extrapolated per nonzero (the oracle's restatement where the
-    reference library is absent)."""
+    baseline: the oracle's restatement of ptiOmpMTTKRP (src/sptensor/
+    mttkrp_omp.c:125-252, nonzero-parallel with atomics) on the first
+    cpu_sample_nnz nonzeros, extrapolated per nonzero."""
     import torch
-    from hiparti_amd import _ffi as F
     from hiparti_amd.device import DevCOO
     N = len(ndims)
     nnz = X.nnz()
@@ -344,33 +321,14 @@ def coo_leg(X, ndims, dfacs, facs_np, outs, R, args, nthreads, rows=None):
     if not args.no_cpu:
         t0 = time.time()
         z = int(min(args.cpu_sample_nnz, nnz))
-        if os.path.exists(REF_LIB):
-            L = C.CDLL(REF_LIB)
-            L.ptiOmpMTTKRP.argtypes = [C.POINTER(F.ptiSparseTensor), C.POINTER(C.POINTER(F.ptiMatrix)),
-                                       C.POINTER(F.ptiIndex), F.ptiIndex, C.c_int]
-            L.ptiOmpMTTKRP.restype = C.c_int
-            view = F.ptiSparseTensor.from_buffer_copy(X.address)   # the first z nonzeros
-            view.nnz = z
-            mats = [F.ptiMatrix(f.shape[0], f.shape[1], f.shape[0], f.shape[1],
-                                f.ctypes.data_as(C.POINTER(F.ptiValue))) for f in facs_np]
-            out = np.zeros((max(ndims), R), np.float32)
-            mats.append(F.ptiMatrix(max(ndims), R, max(ndims), R, out.ctypes.data_as(C.POINTER(F.ptiValue))))
-            ptrs = (C.POINTER(F.ptiMatrix) * (N + 1))(*[C.pointer(m) for m in mats])
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        inds = X.inds()[:, :z].copy()
+        vals = X.values()[:z].copy()
 
-            def run(mode):
-                order = (F.ptiIndex * N)(*([mode] + [(mode + i) % N for i in range(1, N)]))
-                mats[N].nrows = ndims[mode]
-                assert L.ptiOmpMTTKRP(C.byref(view), ptrs, order, mode, int(nthreads)) == 0
-            kind, who = "reference", "reference ptiOmpMTTKRP"
-        else:
-            sys.path.insert(0, os.path.join(ROOT, "oracle"))
-            import oracle as O
-            inds = X.inds()[:, :z].copy()
-            vals = X.values()[:z].copy()
-
-            def run(mode):
-                O.mttkrp(inds, vals, facs_np, mode, kind="omp", nthreads=nthreads)
-            kind, who = "port", "oracle restatement of ptiOmpMTTKRP"
+        def run(mode):
+            O.mttkrp(inds, vals, facs_np, mode, kind="omp", nthreads=nthreads)
+        kind, who = "port", "oracle restatement of ptiOmpMTTKRP"
         total = _time_modes(run, N)
         res["cpu_baseline"] = {"value": round(total / z * nnz * 1e3, 1),
                                "unit": "ms per step (3-mode COO MTTKRP, extrapolated per nonzero)",
@@ -378,6 +336,7 @@ def coo_leg(X, ndims, dfacs, facs_np, outs, R, args, nthreads, rows=None):
                                "sample": "%s (tk=%d) on the first %d nonzeros of the same tensor, all 3 modes, "
                                          "1 warm-up + 2 timed calls per mode" % (who, nthreads, z)}
         res["cpu_baseline"].update(host_facts(nthreads))
+        res["cpu_baseline"]["port_vs_reference"] = PORT_VS_REFERENCE["coo"]
         log("coo cpu baseline %.1f s" % (time.time() - t0))
     return {"coo": res}, got_rows
 
@@ -429,7 +388,7 @@ def precision_leg(args, nthreads):
     Per mode: FP32 versus BF16-factor MTTKRP time (HIP events), the HiCOO
     roofline of each (s_m = 2 for the gathered BF16 factors), the error of
     each against the oracle's FP64 rows (BF16: on the rounded factors), and
-    the reference's ptiOmpMTTKRPHiCOO on a sample."""
+    the oracle's restatement of ptiOmpMTTKRPHiCOO on a sample."""
     import torch
     from hiparti_amd import COOTensor, DenseMatrix
     from hiparti_amd.device import DevHiCOO
@@ -497,7 +456,7 @@ def precision_leg(args, nthreads):
     if not args.no_cpu:
         t0 = time.time()
         cpu_ms, kind, z_s, k_s = cpu_baseline(H, ndims, facs_np, R, args.cpu_sample_nnz, nthreads)
-        who = "reference ptiOmpMTTKRPHiCOO" if kind == "reference" else "oracle restatement of ptiOmpMTTKRPHiCOO"
+        who = "oracle restatement of ptiOmpMTTKRPHiCOO"
         res["cpu_baseline"] = {"value": round(cpu_ms, 1),
                                "unit": "ms per step (4-mode FP32 HiCOO MTTKRP, extrapolated per nonzero)",
                                "cores": nthreads, "kind": kind,
@@ -505,6 +464,7 @@ def precision_leg(args, nthreads):
                                          "tensor, all 4 modes, 1 warm-up + 2 timed calls per mode"
                                          % (who, nthreads, k_s, z_s)}
         res["cpu_baseline"].update(host_facts(nthreads))
+        res["cpu_baseline"]["port_vs_reference"] = PORT_VS_REFERENCE["hicoo"]
         log("precision cpu baseline %.1f s" % (time.time() - t0))
     H.free()
     return {"precision": res}
@@ -829,12 +789,13 @@ def main():
     if not args.no_cpu:
         t0 = time.time()
         cpu_ms, kind, z_s, k_s = cpu_baseline(H, ndims, facs_np, R, args.cpu_sample_nnz, nthreads)
-        who = "reference ptiOmpMTTKRPHiCOO" if kind == "reference" else "oracle restatement of ptiOmpMTTKRPHiCOO"
+        who = "oracle restatement of ptiOmpMTTKRPHiCOO"
         cpu = {"value": round(cpu_ms, 1), "unit": "ms per step (3-mode HiCOO MTTKRP, extrapolated per nonzero)",
                "cores": nthreads, "kind": kind,
                "sample": "%s (tk=%d, tb=1) on the first %d superblocks = %d nonzeros of the same tensor, all 3 "
                          "modes, 1 warm-up + 2 timed calls per mode" % (who, nthreads, k_s, z_s)}
         cpu.update(host_facts(nthreads))
+        cpu["port_vs_reference"] = PORT_VS_REFERENCE["hicoo"]
         log("cpu baseline %.1f s" % (time.time() - t0))
 
     line = {

@@ -250,7 +250,13 @@ def test_cp_als_order5_c4_shape_follows_fp64_oracle(bf16_iters, oracle):
     tensor of the CP-ALS configuration's shape against the oracle's FP64 ALS
     from the same factors over 6 iterations, all-FP32 and with the first 3
     iterations' MTTKRPs on BF16 factor copies (the oracle rounds its factor
-    copies the same way)."""
+    copies the same way).  Fits must agree within 1e-4 either way.  Lambda
+    and the factors must agree within 1e-3 for the all-FP32 run; with BF16
+    copies the bar is 1e-2: an FP32-versus-FP64 difference of ~1e-7 in a
+    factor entry can round it to the neighbouring BF16 value (a 2^-8 relative
+    step) on one side only, and ALS on this tensor, which has no low-rank
+    structure, carries such steps into lambda (measured 1.3e-3 after 6
+    iterations while the fits agreed to 5e-8)."""
     R = 16
     X = c4_tensor(1_500_000, 44)
     inds, vals = X.inds(), X.values()
@@ -266,29 +272,40 @@ def test_cp_als_order5_c4_shape_follows_fp64_oracle(bf16_iters, oracle):
     print("order-5 CP-ALS, bf16_iters=%d: device fits %s, oracle fits %s" % (bf16_iters, res.fits, fits), flush=True)
     assert res.iters == 6
     assert np.allclose(res.fits, fits, atol=1e-4, rtol=0), (res.fits, fits)
-    assert rel_fro(res.lambda_.cpu().numpy(), lam) <= 1e-3
+    tol = 1e-3 if bf16_iters == 0 else 1e-2
+    assert rel_fro(res.lambda_.cpu().numpy(), lam) <= tol
     for f, g in zip(res.factors, fs):
-        assert rel_fro(f.cpu().numpy(), g) <= 1e-3
+        assert rel_fro(f.cpu().numpy(), g) <= tol
 
 
-def test_bf16_then_fp32_reaches_the_fp32_fit_on_a_planted_order5_tensor():
+@pytest.mark.parametrize("seed", [8, 10])
+def test_bf16_then_fp32_reaches_the_fp32_fit_on_a_planted_order5_tensor(seed):
     """Every entry of a 12 x 10 x 9 x 8 x 6 tensor observed, values from a
-    planted rank-3 model: all-FP32 ALS and the schedule with 10 BF16
-    iterations first must end at the same fit (within 1e-4) after 60
-    iterations, both near 1."""
+    planted rank-3 model (factors U(0.2, 1)) plus 5% Gaussian noise, ALS
+    started from the planted factors perturbed by 30%: all-FP32 ALS and the
+    schedule with 10 BF16 iterations first must end at the same fit (within
+    1e-4) after 60 iterations, both at the noise level (fit ~0.95).  The noise
+    keeps the residual norm well above the fit formula's cancellation floor
+    (sqrt(||X||^2 + ||K||^2 - 2<X,K>) is only good to ~sqrt(eps) * ||X|| when
+    the model is exact), and the warm start keeps ALS out of the poor local
+    optima that random starts reach on such small tensors."""
     shape = (12, 10, 9, 8, 6)
-    rng = np.random.default_rng(8)
-    P = [rng.standard_normal((d, 3)) for d in shape]
+    rng = np.random.default_rng(seed)
+    P = [rng.uniform(0.2, 1.0, (d, 3)) for d in shape]
     inds = np.array(np.unravel_index(np.arange(int(np.prod(shape))), shape), np.uint32)
     prod = np.ones((inds.shape[1], 3))
     for m, p in enumerate(P):
         prod *= p[inds[m]]
-    vals = prod.sum(axis=1).astype(np.float32)
+    clean = prod.sum(axis=1)
+    noise = 0.05 * np.linalg.norm(clean) / np.sqrt(clean.size)
+    vals = (clean + noise * rng.standard_normal(clean.size)).astype(np.float32)
+    init = [torch.from_numpy((p * (1 + 0.3 * rng.standard_normal(p.shape))).astype(np.float32)).cuda()
+            for p in P]
     dev, _keep = device_tensor("hicoo", list(shape), inds, vals)
-    r32 = D.cp_als(dev, 3, niters=60, tol=0.0, seed=2)
-    rmix = D.cp_als(dev, 3, niters=60, tol=0.0, seed=2, bf16_iters=10)
-    print("planted order 5: FP32 fit %.8f, BF16-then-FP32 fit %.8f" % (r32.fit, rmix.fit), flush=True)
-    assert r32.fit > 0.999
+    r32 = D.cp_als(dev, 3, niters=60, tol=0.0, factors=[f.clone() for f in init])
+    rmix = D.cp_als(dev, 3, niters=60, tol=0.0, factors=[f.clone() for f in init], bf16_iters=10)
+    print("planted order 5 + 5%% noise: FP32 fit %.8f, BF16-then-FP32 fit %.8f" % (r32.fit, rmix.fit), flush=True)
+    assert 0.94 < r32.fit < 0.96
     assert abs(rmix.fit - r32.fit) <= 1e-4, (rmix.fit, r32.fit)
 
 
