@@ -161,6 +161,7 @@ SIGNATURES = {
     "ptiDevMTTKRPCOO": (C.c_int, [_vp, _P(_vp), ptiIndex, ptiIndex, _P(ptiIndex), ptiIndex, _vp]),
     "ptiDevMTTKRPCOOValues": (C.c_int, [_vp, _vp, _P(_vp), ptiIndex, ptiIndex, _P(ptiIndex), ptiIndex, _vp]),
     "ptiDevCOOCompletionForward": (C.c_int, [_vp, _P(_vp), ptiIndex, ptiIndex, _vp, _vp, _vp]),
+    "ptiDevCOOCompletionForwardNoAlloc": (C.c_int, [_vp, _P(_vp), ptiIndex, ptiIndex, _vp, _vp, _vp]),
     "ptiDevCOOCompletionGrad": (C.c_int, [_vp, _vp, _P(_vp), ptiIndex, ptiIndex, ptiIndex, _vp]),
     "ptiDevCOOCompletionFused": (C.c_int, [_vp, ptiIndex, ptiIndex]),
     "ptiDevCOOCompletionLossGrad": (C.c_int, [_vp, _P(_vp), _P(_vp), ptiIndex, ptiIndex, _vp, _vp]),

@@ -20,6 +20,15 @@
 #include "dropin.hpp"
 #include "launch.hpp"
 
+// Scratch of ptiDevCOOCompletionForwardNoAlloc: the CP-completion forward's
+// per-workgroup FP64 partial sums (one per 1024 nonzeros), so at most
+// NOALLOC_MAX_PARTIALS * 1024 = 2^27 nonzeros.  Every GPU the library runs on
+// holds its own copy (1 MiB) until the process ends.  In host code `scratch`
+// names a host-side stand-in: its device address comes from
+// hipGetSymbolAddress (noalloc_scratch).
+constexpr uint32_t NOALLOC_MAX_PARTIALS = 1u << 17;
+static __device__ __align__(256) double scratch[NOALLOC_MAX_PARTIALS];
+
 struct ptiDevCOO {
     uint32_t nmodes = 0;
     uint64_t nnz = 0;
@@ -428,6 +437,27 @@ int completion_forward(const ptiDevCOO *t, float *const dfacs[], ptiIndex R, pti
     return 0;
 }
 
+// Device address of `scratch` on the current device (*out), looked up once
+// per device with that device current: each GPU holds its own copy at its
+// own address.
+int noalloc_scratch(double **out, const char *M) {
+    constexpr int MAX_DEVICES = 64;
+    static std::mutex mu;
+    static double *addr[MAX_DEVICES] = {};
+    int dev = 0;
+    PTI_HIP(hipGetDevice(&dev), M);
+    if (dev < 0 || dev >= MAX_DEVICES) PTI_FAIL(PTIERR_VALUE_ERROR, M, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        PTI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), M);
+        if (!p) PTI_FAIL(PTIERR_CUDA_ERROR, M, "no device address for the scratch array");
+        addr[dev] = static_cast<double *>(p);
+    }
+    *out = addr[dev];
+    return 0;
+}
+
 void coo_view(const ptiDevCOO *t, uint32_t *nmodes, const ptiIndex **ndims, uint64_t *nnz, const float **vals) {
     *nmodes = t->nmodes;
     *ndims = t->ndims;
@@ -493,6 +523,19 @@ int ptiDevCOOCompletionForward(ptiDevCOO *t, float *const dfacs[], ptiIndex R, p
         PTI_HIP(t->partials.alloc((size_t)std::max<uint32_t>(nblk, 1) * sizeof(double)), M);
     return completion_forward(t, dfacs, R, stride, dresid, dloss, t->partials.as<double>(),
                               static_cast<hipStream_t>(stream), M);
+}
+
+int ptiDevCOOCompletionForwardNoAlloc(ptiDevCOO *t, float *const dfacs[], ptiIndex R, ptiIndex stride,
+                                      float *dresid, double *dloss, void *stream) {
+    static const char *M = "DevCOO Completion (no allocation)";
+    if (R == 0 || R > 256 || stride < R) PTI_FAIL(PTIERR_VALUE_ERROR, M, "rank must be in [1, 256] and <= stride");
+    if (completion_forward_workgroups(t->nnz) > NOALLOC_MAX_PARTIALS)
+        PTI_FAIL(PTIERR_VALUE_ERROR, M, "more than 2^27 nonzeros: beyond the fixed scratch");
+    double *part = nullptr;
+    PTI_CHECK(noalloc_scratch(&part, M), M);
+    // Every workgroup of the forward kernel writes its own partial before the
+    // sum reads it, so the scratch needs no clearing.
+    return completion_forward(t, dfacs, R, stride, dresid, dloss, part, static_cast<hipStream_t>(stream), M);
 }
 
 int ptiRegisterSparseTensor(ptiSparseTensor const *X) {

@@ -76,6 +76,17 @@ int ptiDevMTTKRPCOOValues(ptiDevCOO *t, float const *dvals, float * const dmats[
 int ptiDevCOOCompletionForward(ptiDevCOO *t, float * const dfacs[], ptiIndex R, ptiIndex stride,
                                float *dresid, double *dloss, void *stream);
 
+/* The same forward without any device allocation.  It has no counterpart in
+ * the reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-workgroup partial sums live in a fixed 1 MiB device array (one per
+ * GPU, kept until the process ends), so the tensor may hold at most 2^27
+ * nonzeros (PTIERR_VALUE_ERROR beyond).  Calls that overlap on one GPU share
+ * that array and race; calls on one stream, or ordered across streams, do
+ * not.  Results are bit-identical to ptiDevCOOCompletionForward. */
+int ptiDevCOOCompletionForwardNoAlloc(ptiDevCOO *t, float * const dfacs[], ptiIndex R, ptiIndex stride,
+                                      float *dresid, double *dloss, void *stream);
+
 /* CP-completion gradient (no counterpart in the reference; the backward of
  * the loss above): dmats[nmodes] = MTTKRP of the residuals
  *   e_z = x_z - sum_r prod_m A_m[i_m][r],   A_m = dmats[m] (m < nmodes),
