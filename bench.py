@@ -76,6 +76,8 @@ def parse():
     p.add_argument("--no-precision", action="store_true", help="skip the precision-form (order-4, R=64) leg")
     p.add_argument("--no-errors", action="store_true", help="skip the FP64 accuracy checks")
     p.add_argument("--prec-nnz", type=float, default=200e6)
+    p.add_argument("--prec-cpu-sample-nnz", type=float, default=1e6,
+                   help="nonzeros of the precision tensor the CPU baseline runs on (mostly one thread)")
     p.add_argument("--cpd-cpu-stride", type=int, default=15,
                    help="host CP-ALS baseline runs on every k-th nonzero of the CPD tensor")
     p.add_argument("--cmp-nnz", type=float, default=50e6)
@@ -209,27 +211,40 @@ PORT_VS_REFERENCE = {"hicoo": 1.03, "coo": 1.02}
 
 
 def _sample_prefix(h, sample_nnz):
-    """Smallest prefix of superblocks holding >= sample_nnz nonzeros:
-    (k_end superblocks, b_end blocks, z_end nonzeros)."""
+    """The leading blocks holding about sample_nnz nonzeros, as superblocks:
+    whole superblocks while they fit, then the first blocks of the next one
+    (a superblock can hold most of a tensor whose first mode is Zipf and not
+    relabelled).  Returns (kptr of the sample, b_end blocks, z_end nonzeros,
+    whether the last superblock was cut)."""
     a = h.address
     nk_total = int(a.kptr.len) - 1
     kptr = np.ctypeslib.as_array(a.kptr.data, shape=(nk_total + 1,))
     bptr_all = np.ctypeslib.as_array(a.bptr.data, shape=(int(a.bptr.len),))
-    k_end = int(np.searchsorted(bptr_all[kptr], sample_nnz))
-    k_end = max(1, min(k_end, nk_total))
-    b_end = int(kptr[k_end])
-    return k_end, b_end, int(bptr_all[b_end])
+    k_full = int(np.searchsorted(bptr_all[kptr], sample_nnz, side="right")) - 1   # superblocks that fit whole
+    k_full = max(0, min(k_full, nk_total))
+    sk = list(kptr[:k_full + 1])
+    cut = False
+    if k_full < nk_total and bptr_all[kptr[k_full]] < sample_nnz:
+        nb_total = int(kptr[nk_total])
+        b_end = int(np.searchsorted(bptr_all, sample_nnz))
+        b_end = max(int(kptr[k_full]) + 1, min(b_end, int(kptr[k_full + 1]), nb_total))
+        cut = b_end < int(kptr[k_full + 1])
+        sk.append(b_end)
+    b_end = int(sk[-1])
+    return np.array(sk, np.uint64), b_end, int(bptr_all[b_end]), cut
 
 
-def _time_modes(run, N):
-    """One warm-up and two timed calls per mode; seconds per three-mode pass."""
+def _time_modes(run, N, what="cpu baseline"):
+    """One warm-up and two timed calls per mode; seconds per N-mode pass."""
     total = 0.0
     for mode in range(N):
         run(mode)
         t0 = time.perf_counter()
         for _ in range(2):
             run(mode)
-        total += (time.perf_counter() - t0) / 2
+        dt = (time.perf_counter() - t0) / 2
+        log("%s: mode %d %.3f s per call" % (what, mode, dt))
+        total += dt
     return total
 
 
@@ -241,15 +256,15 @@ def cpu_baseline(h, ndims, facs_np, R, sample_nnz, nthreads):
     tk = host threads and tb = 1 (superblock-parallel with per-element
     atomics, src/sptensor/hicoo/mttkrp_omp.c:184-204,400-467); the reference
     library itself is built only where its sources are, to pin that
-    restatement (PORT_VS_REFERENCE).  Returns (ms, kind, z_end, k_end)."""
+    restatement (PORT_VS_REFERENCE).  Returns (ms, kind, z_end, a text naming
+    the sample)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     a = h.address
     N = len(ndims)
-    k_end, b_end, z_end = _sample_prefix(h, sample_nnz)
-    kptr = np.ctypeslib.as_array(a.kptr.data, shape=(int(a.kptr.len),))
+    skptr, b_end, z_end, cut = _sample_prefix(h, sample_nnz)
     sub = {
-        "kptr": kptr[:k_end + 1].copy(),
+        "kptr": skptr,
         "bptr": np.ctypeslib.as_array(a.bptr.data, shape=(b_end + 1,)).copy(),
         "binds": np.stack([np.ctypeslib.as_array(a.binds[m].data, shape=(b_end,)) for m in range(N)]),
         "einds": np.stack([np.ctypeslib.as_array(a.einds[m].data, shape=(z_end,)) for m in range(N)]),
@@ -258,8 +273,15 @@ def cpu_baseline(h, ndims, facs_np, R, sample_nnz, nthreads):
 
     def run(mode):
         O.mttkrp_hicoo(sub, int(a.sb_bits), facs_np, mode, kind="omp", nthreads=nthreads)
-    total = _time_modes(run, N)
-    return total / z_end * int(a.nnz) * 1e3, "port", z_end, k_end
+    total = _time_modes(run, N, "HiCOO cpu baseline")
+    nk = len(skptr) - 1
+    if cut:
+        text = ("%d whole superblocks and the first %d blocks of the next (%d of its %d blocks)"
+                % (nk - 1, b_end - int(skptr[-2]), b_end - int(skptr[-2]),
+                   int(np.ctypeslib.as_array(a.kptr.data, shape=(int(a.kptr.len),))[nk] - skptr[-2])))
+    else:
+        text = "the first %d superblocks" % nk
+    return total / z_end * int(a.nnz) * 1e3, "port", z_end, text
 
 
 def coo_leg(X, ndims, dfacs, facs_np, outs, R, args, nthreads, rows=None):
@@ -455,14 +477,16 @@ def precision_leg(args, nthreads):
         log("precision errors in %.1f s: fp32 %s bf16 %s" % (time.time() - t0, e32, e16))
     if not args.no_cpu:
         t0 = time.time()
-        cpu_ms, kind, z_s, k_s = cpu_baseline(H, ndims, facs_np, R, args.cpu_sample_nnz, nthreads)
+        cpu_ms, kind, z_s, where = cpu_baseline(H, ndims, facs_np, R, args.prec_cpu_sample_nnz, nthreads)
         who = "oracle restatement of ptiOmpMTTKRPHiCOO"
         res["cpu_baseline"] = {"value": round(cpu_ms, 1),
                                "unit": "ms per step (4-mode FP32 HiCOO MTTKRP, extrapolated per nonzero)",
                                "cores": nthreads, "kind": kind,
-                               "sample": "%s (tk=%d, tb=1) on the first %d superblocks = %d nonzeros of the same "
-                                         "tensor, all 4 modes, 1 warm-up + 2 timed calls per mode"
-                                         % (who, nthreads, k_s, z_s)}
+                               "sample": "%s (tk=%d, tb=1) on %s = %d nonzeros of the same tensor, all 4 modes, "
+                                         "1 warm-up + 2 timed calls per mode; the first superblock holds most of "
+                                         "the nonzeros (mode 0 is Zipf(1.2), not relabelled), and the reference's "
+                                         "superblock-parallel schedule runs it on one thread"
+                                         % (who, nthreads, where, z_s)}
         res["cpu_baseline"].update(host_facts(nthreads))
         res["cpu_baseline"]["port_vs_reference"] = PORT_VS_REFERENCE["hicoo"]
         log("precision cpu baseline %.1f s" % (time.time() - t0))
@@ -788,12 +812,12 @@ def main():
     cpu = None
     if not args.no_cpu:
         t0 = time.time()
-        cpu_ms, kind, z_s, k_s = cpu_baseline(H, ndims, facs_np, R, args.cpu_sample_nnz, nthreads)
+        cpu_ms, kind, z_s, where = cpu_baseline(H, ndims, facs_np, R, args.cpu_sample_nnz, nthreads)
         who = "oracle restatement of ptiOmpMTTKRPHiCOO"
         cpu = {"value": round(cpu_ms, 1), "unit": "ms per step (3-mode HiCOO MTTKRP, extrapolated per nonzero)",
                "cores": nthreads, "kind": kind,
-               "sample": "%s (tk=%d, tb=1) on the first %d superblocks = %d nonzeros of the same tensor, all 3 "
-                         "modes, 1 warm-up + 2 timed calls per mode" % (who, nthreads, k_s, z_s)}
+               "sample": "%s (tk=%d, tb=1) on %s = %d nonzeros of the same tensor, all 3 modes, 1 warm-up + 2 "
+                         "timed calls per mode" % (who, nthreads, where, z_s)}
         cpu.update(host_facts(nthreads))
         cpu["port_vs_reference"] = PORT_VS_REFERENCE["hicoo"]
         log("cpu baseline %.1f s" % (time.time() - t0))
