@@ -86,10 +86,13 @@ struct Ptr8 {
 // blocks of 2^ROW_BITS rows, chunks and units (one wave per chunk): entries in
 // the StreamShape layout for ng lane groups, the coordinates of the other
 // modes in ascending mode order, the value bits and the row inside its block.
+// perm (optional): the entries of each block in another order (positions
+// inside the block's range of the sorted copy).
 constexpr uint32_t ROW_BITS = 7;
 __global__ void __launch_bounds__(256) coo_stream_fill_kernel(Ptr8 sinds, const float *__restrict__ svals,
                                                               const uint32_t *__restrict__ chunk_z, uint32_t nchunks,
                                                               uint32_t N, uint32_t mode, uint32_t ng,
+                                                              const uint32_t *__restrict__ perm,
                                                               uint4 *__restrict__ ent) {
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t c = blockIdx.x * 4 + wid;
@@ -104,13 +107,31 @@ __global__ void __launch_bounds__(256) coo_stream_fill_kernel(Ptr8 sinds, const 
         for (uint32_t k = 0; k < 12; ++k) w[k] = 0u;
         w[N] = DUMMY_ROW;
         if (j < per && g * per + j < cnt) {
-            const uint64_t z = (uint64_t)z0 + g * per + j;
+            const uint64_t zs = (uint64_t)z0 + g * per + j;
+            const uint64_t z = perm ? perm[zs] : zs;
             for (uint32_t i = 0; i + 1 < N; ++i) w[i] = sinds.p[i < mode ? i : i + 1][z];
             w[N - 1] = __float_as_uint(svals[z]);
             w[N] = sinds.p[mode][z] & ((1u << ROW_BITS) - 1u);
         }
         for (uint32_t q = 0; q < NP; ++q)
             ent[((uint64_t)c * NP + q) * CHUNK + e] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+    }
+}
+
+// Windowed entry order of a long row block: rows of the largest gathered mode
+// per window (4096 rows: 512 KB of an FP32 factor at rank 32).
+constexpr uint32_t WINDOW_BITS = 12;
+
+// Sort keys of the windowed order: (row block, window of the gathered
+// coordinate `sec`, row inside the block), over the copy sorted by `mode`.
+__global__ void __launch_bounds__(256) window_keys_kernel(const uint32_t *__restrict__ rows,
+                                                          const uint32_t *__restrict__ sec, uint64_t n, uint32_t wbits,
+                                                          uint64_t *__restrict__ key, uint32_t *__restrict__ pos) {
+    for (uint64_t z = (uint64_t)blockIdx.x * 256 + threadIdx.x; z < n; z += (uint64_t)gridDim.x * 256) {
+        const uint32_t r = rows[z];
+        key[z] = ((uint64_t)(r >> ROW_BITS) << (wbits + ROW_BITS)) | ((uint64_t)(sec[z] >> WINDOW_BITS) << ROW_BITS) |
+                 (r & ((1u << ROW_BITS) - 1u));
+        pos[z] = (uint32_t)z;
     }
 }
 
@@ -223,11 +244,49 @@ int ensure_coo_stream(ptiDevCOO *t, ptiIndex mode, uint32_t ng, hipStream_t s, c
     PTI_HIP(t->stream[mode].alloc((size_t)u.nchunks * NP * CHUNK * sizeof(uint4)), M);
     Ptr8 p{};
     for (uint32_t m = 0; m < N; ++m) p.p[m] = t->sinds[mode][m].as<uint32_t>();
+    // Row blocks holding two or more units' worth of nonzeros on average are
+    // ordered by windows of the largest gathered mode's coordinate (then by
+    // row), and units are dealt by their rank inside their block: the units
+    // in flight then gather that factor from a few windows, which stay in L2,
+    // instead of sweeping all of it each.
+    uint32_t sec = mode == 0 ? 1 : 0;
+    for (uint32_t m = 0; m < N; ++m)
+        if (m != mode && t->ndims[m] > t->ndims[sec]) sec = m;
+    const uint64_t nblocks = ((uint64_t)t->ndims[mode] + (1u << ROW_BITS) - 1) >> ROW_BITS;
+    const bool windowed = t->nnz >= nblocks * 2 * UNIT_CHUNKS * CHUNK && t->ndims[sec] >= (4u << WINDOW_BITS);
+    Buffer key, pos, skey, spos, tmp;
+    if (windowed) {
+        const uint64_t n = t->nnz;
+        unsigned bbits = 0, wbits = 0;
+        while (bbits < 32 && ((nblocks - 1) >> bbits)) ++bbits;
+        while (wbits < 32 && (((uint64_t)t->ndims[sec] - 1) >> WINDOW_BITS) >> wbits) ++wbits;
+        PTI_HIP(key.alloc(n * 8), M);
+        PTI_HIP(skey.alloc(n * 8), M);
+        PTI_HIP(pos.alloc(n * 4), M);
+        PTI_HIP(spos.alloc(n * 4), M);
+        hipLaunchKernelGGL(window_keys_kernel, grid_1d(n), dim3(256), 0, s, t->sinds[mode][mode].as<uint32_t>(),
+                           t->sinds[mode][sec].as<uint32_t>(), n, wbits, key.as<uint64_t>(), pos.as<uint32_t>());
+        PTI_HIP(hipGetLastError(), M);
+        const unsigned end_bit = bbits + wbits + ROW_BITS;
+        size_t tb = 0;
+        PTI_HIP(rocprim::radix_sort_pairs(nullptr, tb, key.as<uint64_t>(), skey.as<uint64_t>(), pos.as<uint32_t>(),
+                                          spos.as<uint32_t>(), (size_t)n, 0u, end_bit, s),
+                M);
+        PTI_HIP(tmp.alloc(tb), M);
+        PTI_HIP(rocprim::radix_sort_pairs(tmp.ptr, tb, key.as<uint64_t>(), skey.as<uint64_t>(), pos.as<uint32_t>(),
+                                          spos.as<uint32_t>(), (size_t)n, 0u, end_bit, s),
+                M);
+    }
     hipLaunchKernelGGL(coo_stream_fill_kernel, dim3((u.nchunks + 3) / 4), dim3(256), 0, s, p,
                        t->svals[mode].as<float>(), u.chunk_z.as<uint32_t>(), u.nchunks, N, (uint32_t)mode, ng,
-                       t->stream[mode].as<uint4>());
+                       windowed ? spos.as<uint32_t>() : nullptr, t->stream[mode].as<uint4>());
     PTI_HIP(hipGetLastError(), M);
-    PTI_CHECK(build_unit_order(u.unit_chunk.as<uint32_t>(), u.nunits, t->unit_order[mode], s, M), M);
+    if (windowed)
+        PTI_CHECK(build_rank_unit_order(u, t->unit_order[mode], s, M), M);
+    else
+        PTI_CHECK(build_unit_order(u.unit_chunk.as<uint32_t>(), u.nunits, t->unit_order[mode], s, M), M);
+    // The temporaries are released on return: the queued work must finish.
+    PTI_HIP(hipStreamSynchronize(s), M);
     t->stream_ng[mode] = ng;
     return 0;
 }

@@ -29,6 +29,7 @@
 // (N + 1)-word entries laid out for the lane groups of the rank shape
 // (StreamShape in mttkrp_kernels.hpp), 4 KB per chunk and 16-byte plane, read
 // by hicoo_stream_kernel.
+#include <algorithm>
 #include <atomic>
 #include <mutex>
 #include <unordered_map>
@@ -41,8 +42,6 @@
 #include "launch.hpp"
 
 namespace {
-constexpr uint32_t UNIT_CHUNKS = 64;  // chunks (of 256 nonzeros) per workgroup unit
-
 struct DimArgs {
     uint32_t d[8];
 };
@@ -697,10 +696,15 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
         Shape sh;
         if (!choose_shape(Rt, stride, out_stride, sh)) PTI_FAIL(PTIERR_VALUE_ERROR, M, "unsupported rank tile");
         // BF16 factors on the stream kernel: eight ranks (16 bytes) per lane
-        // when the rank tile fills at least four lanes that way, so that one
-        // gather instruction covers twice the rows of the 8-byte shape.
-        if (bf16 && fits && Rt % 8 == 0 && stride % 8 == 0 && Rt >= 32) {
-            Shape w{8, Rt / 8 <= 4 ? 4 : 8};
+        // when the rank tile needs more than four such lanes (lane groups of
+        // eight), so that one gather instruction covers twice the rows of the
+        // 8-byte shape.  At 32 ranks and below the 16-byte shape would need
+        // groups of four lanes, sixteen slices per chunk: measured on the
+        // 100M-nonzero Zipf tensor at rank 32, their run-sum flushes into the
+        // LDS tile (eight atomics per lane each) cost more than the halved
+        // gathers save (1.26 against 1.07 ms per mode).
+        if (bf16 && fits && Rt % 8 == 0 && stride % 8 == 0 && Rt > 32) {
+            Shape w{8, 8};
             if (stream_supported(N, w.G)) sh = w;
         }
         // A narrower last tile keeps the first tile's lane-group width G, with
@@ -788,6 +792,27 @@ int build_unit_order(const uint32_t *unit_chunk, uint32_t nunits, Buffer &order,
                                            order.as<uint32_t>(), (size_t)nunits, 0, 8, s),
             M);
     // The temporaries are released on return.
+    PTI_HIP(hipStreamSynchronize(s), M);
+    return 0;
+}
+
+// Units in order of their rank inside their block-row (all first units, then
+// all second units, ...), block-rows ascending within a rank: for stream forms
+// whose block-rows walk a gathered factor window by window, the units in
+// flight together then read the same windows.
+int build_rank_unit_order(const RowUnits &u, Buffer &order, hipStream_t s, const char *M) {
+    PTI_HIP(order.alloc((size_t)u.nunits * 4), M);
+    if (u.nunits == 0) return 0;
+    std::vector<uint32_t> br(u.nunits), rank(u.nunits), ord(u.nunits);
+    PTI_HIP(hipMemcpyAsync(br.data(), u.unit_br.ptr, (size_t)u.nunits * 4, hipMemcpyDeviceToHost, s), M);
+    PTI_HIP(hipStreamSynchronize(s), M);
+    for (uint32_t i = 0; i < u.nunits; ++i) {
+        rank[i] = (i > 0 && br[i] == br[i - 1]) ? rank[i - 1] + 1 : 0;
+        ord[i] = i;
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return rank[a] < rank[b]; });
+    PTI_HIP(hipMemcpyAsync(order.ptr, ord.data(), (size_t)u.nunits * 4, hipMemcpyHostToDevice, s), M);
+    // The host vector must outlive the copy.
     PTI_HIP(hipStreamSynchronize(s), M);
     return 0;
 }

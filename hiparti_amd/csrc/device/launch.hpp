@@ -50,6 +50,8 @@ inline uint32_t coords_fit_u24(const uint32_t *ndims, uint32_t N, uint32_t mode,
     return 1;
 }
 
+constexpr uint32_t UNIT_CHUNKS = 64;  // chunks (of 256 nonzeros) per workgroup unit
+
 // A sorted copy's rows cut into 2^sb-row blocks, chunks and units, the
 // structure of a HiCOO mode copy (build_row_units in hicoo_dev.hip).
 struct RowUnits {
@@ -58,6 +60,7 @@ struct RowUnits {
 };
 int build_row_units(const uint32_t *rows, uint64_t n, uint32_t sb, RowUnits &u, hipStream_t s, const char *M);
 int build_unit_order(const uint32_t *unit_chunk, uint32_t nunits, Buffer &order, hipStream_t s, const char *M);
+int build_rank_unit_order(const RowUnits &u, Buffer &order, hipStream_t s, const char *M);
 
 // Shapes the stream kernel runs: orders up to 5 (beyond, the entry planes and
 // the gathered rows no longer fit the register budget) and lane groups of at

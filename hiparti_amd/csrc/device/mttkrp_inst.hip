@@ -181,12 +181,8 @@ hipError_t PTI_CAT(launch_hicoo_, PTI_NMODES)(const HiCOOArgs &h, const FactorAr
 hipError_t PTI_CAT(launch_stream_, PTI_NMODES)(const StreamArgs &h, const FactorArgs &a, Shape sh, bool bf16,
                                                hipStream_t s) {
     switch (sh.V) {
-        case 8:
-            switch (sh.G) {
-                case 4: return stream_vg<8, 4>(h, a, bf16, s);
-                case 8: return stream_vg<8, 8>(h, a, bf16, s);
-                default: return hipErrorInvalidValue;
-            }
+        case 8:   // BF16 rank tiles above 32 ranks: lane groups of eight
+            return sh.G == 8 ? stream_vg<8, 8>(h, a, bf16, s) : hipErrorInvalidValue;
         case 4: return stream_v<4>(h, a, sh.G, bf16, s);
         case 2: return stream_v<2>(h, a, sh.G, bf16, s);
         case 1: return stream_v<1>(h, a, sh.G, bf16, s);

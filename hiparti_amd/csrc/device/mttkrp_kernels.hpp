@@ -953,50 +953,75 @@ __global__ void __launch_bounds__(256) coo_mttkrp_kernel(COOArgs c, FactorArgs a
     process_chunk<N, V, G, FT>(a, st, cnt, GlobalSink{a.out, a.out_stride}, lane);
 }
 
+// Sum of x over each group of G lanes, in every lane of the group: DPP moves
+// for groups of up to 16 lanes, a butterfly of shuffles for wider groups.
+template <int G>
+__device__ __forceinline__ float group_sum_any(float x) {
+    if constexpr (G <= 16) {
+        return group_sum<G>(x);
+    } else {
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        return x;
+    }
+}
+
 // Sampled reconstruction at the nonzeros (CP completion forward):
 //   yhat_z = sum_r prod_m A_m[i_m][r],  resid_z = x_z - yhat_z,
 // and each workgroup's partial sum of resid_z^2 in FP64.  Storage order, one
-// wave per chunk of CHUNK nonzeros, G lanes x V ranks per nonzero, the group
-// stepping over column blocks of G * V ranks when R is wider; the rank sum is
-// a butterfly over the G lanes of a group.
+// wave per chunk of CHUNK nonzeros, taken as blocks of 64: lane l loads the
+// coordinates and the value of nonzero l of the block (N + 1 coalesced loads
+// per 64 nonzeros), and lane group g (G lanes x V ranks) then works through
+// the G nonzeros its own lanes hold, taking each one's coordinates from its
+// lane with group_word (DPP for groups of up to 16 lanes), gathering the N
+// factor rows (stepping over column blocks of G * V ranks when R is wider)
+// and summing over the ranks within the group.  The lane that holds a
+// nonzero keeps its residual, so the block's residuals are written by one
+// coalesced store.  Lanes past the chunk's end repeat the block's first
+// nonzero; their results are neither stored nor counted.
 template <int N, int V, int G>
 __global__ void __launch_bounds__(256) completion_forward_kernel(COOArgs c, FactorArgs a, float *resid,
                                                                  double *partials) {
     __shared__ double wsum[WAVES_PER_BLOCK];
-    constexpr int NG = 64 / G;
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t g = (uint32_t)lane / G;
     const uint32_t col = (uint32_t)(lane % G) * V;
+    const int gbase = (lane & ~(G - 1)) * 4;  // byte address of lane 0 of this lane's group
     const uint64_t chunk = (uint64_t)blockIdx.x * WAVES_PER_BLOCK + wid;
     double sq = 0.0;
     if (chunk < c.nchunks) {
         const uint64_t z0 = chunk * CHUNK;
         const uint64_t rem = c.nnz - z0;
         const uint32_t cnt = rem < CHUNK ? (uint32_t)rem : (uint32_t)CHUNK;
-        const uint32_t iters = (cnt + NG - 1) / NG;  // the same for every group: shuffles stay uniform
-        for (uint32_t it = 0; it < iters; ++it) {
-            const uint32_t p = it * NG + g;
-            const bool valid = p < cnt;
-            const uint64_t z = z0 + (valid ? p : 0);
-            float part = 0.f;
-            if (valid) {
+        for (uint32_t b = 0; b < cnt; b += 64) {
+            const bool mine = b + (uint32_t)lane < cnt;
+            const uint64_t z = z0 + b + (mine ? (uint32_t)lane : 0u);
+            uint32_t w[N];
+#pragma unroll
+            for (int m = 0; m < N; ++m) w[m] = c.inds[m][z];
+            const float x = c.vals[z];
+            float e = 0.f;
+            static_for<0, G>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                uint32_t row[N];
+#pragma unroll
+                for (int m = 0; m < N; ++m) row[m] = group_word<G, j>(w[m], gbase);
+                float part = 0.f;
                 for (uint32_t cc = col; cc < a.R; cc += G * V) {
                     float prod[V], t[V];
-                    FactorLoad<float, V>::load(a.fac[0], (uint64_t)c.inds[0][z] * a.stride + cc, prod);
+                    FactorLoad<float, V>::load(a.fac[0], (uint64_t)row[0] * a.stride + cc, prod);
 #pragma unroll
                     for (int m = 1; m < N; ++m) {
-                        FactorLoad<float, V>::load(a.fac[m], (uint64_t)c.inds[m][z] * a.stride + cc, t);
+                        FactorLoad<float, V>::load(a.fac[m], (uint64_t)row[m] * a.stride + cc, t);
 #pragma unroll
                         for (int k = 0; k < V; ++k) prod[k] *= t[k];
                     }
 #pragma unroll
                     for (int k = 0; k < V; ++k) part += prod[k];
                 }
-            }
-#pragma unroll
-            for (int off = G / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-            if (valid && (lane % G) == 0) {
-                const float e = c.vals[z] - part;
+                part = group_sum_any<G>(part);
+                if (lane % G == j) e = x - part;
+            });
+            if (mine) {
                 resid[z] = e;
                 sq += (double)e * (double)e;
             }
