@@ -116,18 +116,19 @@ def kernel_source_sha():
         return hashlib.sha256(f.read()).hexdigest()
 
 
-def measured_traffic():
-    """HBM bytes per launch of the FP32 HiCOO kernel on this workload, from
-    the committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE,
+def measured_traffic(which="traffic"):
+    """HBM bytes per launch of the stream kernel on this workload, from the
+    committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE,
     calibrated on a plain copy), when it was taken on the current kernel:
-    same kernel name and the same SHA-256 of the kernel source."""
+    same kernel name and the same SHA-256 of the kernel source.  which:
+    "traffic" (FP32 HiCOO), "bf16_traffic" or "coo_traffic"."""
     try:
         text = open(TRAFFIC_FILE).read()
     except OSError:
         return None
     kern = re.search(r"^kernel: (.*)$", text, re.M)
     sha = re.search(r"^kernel_source_sha256: ([0-9a-f]+)$", text, re.M)
-    val = re.search(r"^traffic_bytes_per_launch: (\d+)$", text, re.M)
+    val = re.search(r"^%s_bytes_per_launch: (\d+)$" % which, text, re.M)
     if not kern or not val or not sha or kern.group(1).strip() != KERNEL or sha.group(1) != kernel_source_sha():
         return None
     return int(val.group(1))
@@ -336,7 +337,7 @@ def coo_leg(X, ndims, dfacs, facs_np, outs, R, args, nthreads, rows=None):
     achieved = alg / (np.mean(per_mode) * 1e-3) / 1e9
     res = {"ms_per_step": round(ms, 4), "per_mode_ms": [round(x, 4) for x in per_mode],
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("coo_traffic"),
                         "kernel": "hicoo_stream_kernel<3, 4, 8, float, 8> on the COO copy's 128-row blocks "
                                   "(+ output zero fill)",
                         "alg_bytes_per_launch": int(alg)}}
@@ -774,6 +775,7 @@ def main():
         extra["bf16_per_mode_ms"] = [round(x, 4) for x in bf16_per_mode]
         extra["bf16_hbm_frac"] = round(float(np.mean([b / (t * 1e-3) for b, t in zip(bf_bytes, bf16_per_mode)]))
                                        / 1e9 / HBM_PEAK_GBS, 4)
+        extra["bf16_traffic"] = measured_traffic("bf16_traffic")
 
     gotcoo = None
     if not args.no_coo:
