@@ -118,10 +118,6 @@ __global__ void __launch_bounds__(256) coo_stream_fill_kernel(Ptr8 sinds, const 
     }
 }
 
-// Windowed entry order of a long row block: rows of the largest gathered mode
-// per window (4096 rows: 512 KB of an FP32 factor at rank 32).
-constexpr uint32_t WINDOW_BITS = 12;
-
 // Sort keys of the windowed order: (row block, window of the gathered
 // coordinate `sec`, row inside the block), over the copy sorted by `mode`.
 __global__ void __launch_bounds__(256) window_keys_kernel(const uint32_t *__restrict__ rows,
@@ -244,16 +240,12 @@ int ensure_coo_stream(ptiDevCOO *t, ptiIndex mode, uint32_t ng, hipStream_t s, c
     PTI_HIP(t->stream[mode].alloc((size_t)u.nchunks * NP * CHUNK * sizeof(uint4)), M);
     Ptr8 p{};
     for (uint32_t m = 0; m < N; ++m) p.p[m] = t->sinds[mode][m].as<uint32_t>();
-    // Row blocks holding two or more units' worth of nonzeros on average are
-    // ordered by windows of the largest gathered mode's coordinate (then by
-    // row), and units are dealt by their rank inside their block: the units
-    // in flight then gather that factor from a few windows, which stay in L2,
-    // instead of sweeping all of it each.
+    // Long row blocks take the windowed order (windowed_order).
     uint32_t sec = mode == 0 ? 1 : 0;
     for (uint32_t m = 0; m < N; ++m)
         if (m != mode && t->ndims[m] > t->ndims[sec]) sec = m;
     const uint64_t nblocks = ((uint64_t)t->ndims[mode] + (1u << ROW_BITS) - 1) >> ROW_BITS;
-    const bool windowed = t->nnz >= nblocks * 2 * UNIT_CHUNKS * CHUNK && t->ndims[sec] >= (4u << WINDOW_BITS);
+    const bool windowed = windowed_order(t->nnz, t->ndims[mode], t->ndims[sec], ROW_BITS);
     Buffer key, pos, skey, spos, tmp;
     if (windowed) {
         const uint64_t n = t->nnz;
@@ -282,7 +274,7 @@ int ensure_coo_stream(ptiDevCOO *t, ptiIndex mode, uint32_t ng, hipStream_t s, c
                        windowed ? spos.as<uint32_t>() : nullptr, t->stream[mode].as<uint4>());
     PTI_HIP(hipGetLastError(), M);
     if (windowed)
-        PTI_CHECK(build_rank_unit_order(u, t->unit_order[mode], s, M), M);
+        PTI_CHECK(build_rank_unit_order(u.unit_br.as<uint32_t>(), u.nunits, t->unit_order[mode], s, M), M);
     else
         PTI_CHECK(build_unit_order(u.unit_chunk.as<uint32_t>(), u.nunits, t->unit_order[mode], s, M), M);
     // The temporaries are released on return: the queued work must finish.

@@ -314,9 +314,11 @@ __global__ void __launch_bounds__(256) chunk_sort_kernel(const uint64_t *__restr
 
 // Stream-form build, step 1 (one wave per chunk): for each nonzero of the
 // mode copy its block and its sort key: output row (the target-mode
-// coordinate) in the high word, the coordinate of mode `sec` in the low word.
+// coordinate) in the high word, the coordinate of mode `sec` in the low word;
+// or, with wbits > 0 (windowed order), (block-row, window of `sec`'s
+// coordinate, row inside the block-row, the coordinate's low WINDOW_BITS).
 __global__ void __launch_bounds__(256) stream_keys_kernel(HiCOOArgs h, uint32_t nchunks, uint32_t N, uint32_t mode,
-                                                          uint32_t sec, uint64_t *__restrict__ key,
+                                                          uint32_t sec, uint32_t wbits, uint64_t *__restrict__ key,
                                                           uint32_t *__restrict__ pos, uint32_t *__restrict__ blk_of) {
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t c = blockIdx.x * 4 + wid;
@@ -331,7 +333,14 @@ __global__ void __launch_bounds__(256) stream_keys_kernel(HiCOOArgs h, uint32_t 
             return (h.binds[(uint64_t)b * N + m] << h.sb_bits) +
                    ((h.einds[(uint64_t)z * EW + m / 4] >> ((m % 4) * 8)) & 0xFFu);
         };
-        key[z] = ((uint64_t)coord(mode) << 32) | coord(sec);
+        if (wbits) {
+            const uint32_t row = coord(mode), x = coord(sec), sb = h.sb_bits;
+            key[z] = ((uint64_t)(row >> sb) << (wbits + sb + WINDOW_BITS)) |
+                     ((uint64_t)(x >> WINDOW_BITS) << (sb + WINDOW_BITS)) |
+                     ((uint64_t)(row & ((1u << sb) - 1u)) << WINDOW_BITS) | (x & ((1u << WINDOW_BITS) - 1u));
+        } else {
+            key[z] = ((uint64_t)coord(mode) << 32) | coord(sec);
+        }
         pos[z] = z;
     }
 }
@@ -633,6 +642,7 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
     for (uint32_t m = 0; m < N; ++m)
         if (m != mode && t->ndims[m] >= t->ndims[sec]) sec = m;
     const uint64_t nnz = t->nnz;
+    const bool windowed = windowed_order(nnz, t->ndims[mode], t->ndims[sec], t->sb_bits);
     Buffer key, pos, skey, spos, blk_of, tmp;
     PTI_HIP(key.alloc(nnz * 8), M);
     PTI_HIP(pos.alloc(nnz * 4), M);
@@ -640,12 +650,14 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
     PTI_HIP(spos.alloc(nnz * 4), M);
     PTI_HIP(blk_of.alloc(nnz * 4), M);
     const dim3 grid((c.nchunks + 3) / 4);
+    unsigned rbits = 1, wbits = 0;
+    while (rbits < 32 && ((uint64_t)t->ndims[mode] - 1) >> rbits) ++rbits;
+    if (windowed)
+        while (wbits < 32 && (((uint64_t)t->ndims[sec] - 1) >> WINDOW_BITS) >> wbits) ++wbits;
     hipLaunchKernelGGL(stream_keys_kernel, grid, dim3(256), 0, s, hicoo_args(t, mode), c.nchunks, N, mode, sec,
-                       key.as<uint64_t>(), pos.as<uint32_t>(), blk_of.as<uint32_t>());
+                       windowed ? wbits : 0u, key.as<uint64_t>(), pos.as<uint32_t>(), blk_of.as<uint32_t>());
     PTI_HIP(hipGetLastError(), M);
-    unsigned kbits = 1;
-    while (kbits < 32 && ((uint64_t)t->ndims[mode] - 1) >> kbits) ++kbits;
-    kbits += 32;
+    const unsigned kbits = windowed ? std::max<unsigned>(rbits, t->sb_bits) + wbits + WINDOW_BITS : rbits + 32;
     size_t tb = 0;
     PTI_HIP(rocprim::radix_sort_pairs(nullptr, tb, key.as<uint64_t>(), skey.as<uint64_t>(), pos.as<uint32_t>(),
                                       spos.as<uint32_t>(), (size_t)nnz, 0, kbits, s),
@@ -657,7 +669,12 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
     hipLaunchKernelGGL(stream_fill_kernel, grid, dim3(256), 0, s, hicoo_args(t, mode), c.nchunks, N, mode, ng,
                        spos.as<uint32_t>(), blk_of.as<uint32_t>(), dst.ent.as<uint4>());
     PTI_HIP(hipGetLastError(), M);
-    if (!c.unit_order.ptr) PTI_CHECK(build_unit_order(c.unit_chunk.as<uint32_t>(), c.nunits, c.unit_order, s, M), M);
+    if (!c.unit_order.ptr) {
+        if (windowed)
+            PTI_CHECK(build_rank_unit_order(c.unit_br.as<uint32_t>(), c.nunits, c.unit_order, s, M), M);
+        else
+            PTI_CHECK(build_unit_order(c.unit_chunk.as<uint32_t>(), c.nunits, c.unit_order, s, M), M);
+    }
     // The temporaries are released on return: the queued work must finish.
     PTI_HIP(hipStreamSynchronize(s), M);
     dst.ng = ng;
@@ -796,22 +813,41 @@ int build_unit_order(const uint32_t *unit_chunk, uint32_t nunits, Buffer &order,
     return 0;
 }
 
-// Units in order of their rank inside their block-row (all first units, then
-// all second units, ...), block-rows ascending within a rank: for stream forms
-// whose block-rows walk a gathered factor window by window, the units in
-// flight together then read the same windows.
-int build_rank_unit_order(const RowUnits &u, Buffer &order, hipStream_t s, const char *M) {
-    PTI_HIP(order.alloc((size_t)u.nunits * 4), M);
-    if (u.nunits == 0) return 0;
-    std::vector<uint32_t> br(u.nunits), rank(u.nunits), ord(u.nunits);
-    PTI_HIP(hipMemcpyAsync(br.data(), u.unit_br.ptr, (size_t)u.nunits * 4, hipMemcpyDeviceToHost, s), M);
+// Units dealt by their rank inside their block-row, for stream forms whose
+// block-rows walk a gathered factor window by window (unit r of every
+// block-row covers about the same windows).  Workgroup b runs on XCD b mod 8
+// (round-robin dealing), so the units of rank r go to the positions of XCD
+// r mod 8, in rank order and then block-row order on each: each XCD's L2 then
+// holds the windows of about one rank at a time instead of those of every
+// rank in flight.  Positions left over by an XCD with fewer units take the
+// next XCD's.
+int build_rank_unit_order(const uint32_t *unit_br, uint32_t nunits, Buffer &order, hipStream_t s, const char *M) {
+    constexpr uint32_t XCDS = 8;
+    PTI_HIP(order.alloc((size_t)nunits * 4), M);
+    if (nunits == 0) return 0;
+    std::vector<uint32_t> br(nunits), rank(nunits), ord;
+    PTI_HIP(hipMemcpyAsync(br.data(), unit_br, (size_t)nunits * 4, hipMemcpyDeviceToHost, s), M);
     PTI_HIP(hipStreamSynchronize(s), M);
-    for (uint32_t i = 0; i < u.nunits; ++i) {
+    std::vector<uint32_t> by_rank(nunits);
+    for (uint32_t i = 0; i < nunits; ++i) {
         rank[i] = (i > 0 && br[i] == br[i - 1]) ? rank[i - 1] + 1 : 0;
-        ord[i] = i;
+        by_rank[i] = i;
     }
-    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return rank[a] < rank[b]; });
-    PTI_HIP(hipMemcpyAsync(order.ptr, ord.data(), (size_t)u.nunits * 4, hipMemcpyHostToDevice, s), M);
+    std::stable_sort(by_rank.begin(), by_rank.end(), [&](uint32_t a, uint32_t b) { return rank[a] < rank[b]; });
+    std::vector<std::vector<uint32_t>> q(XCDS);
+    for (uint32_t i : by_rank) q[rank[i] % XCDS].push_back(i);
+    std::vector<size_t> head(XCDS, 0);
+    ord.reserve(nunits);
+    for (uint32_t b = 0; b < nunits; ++b) {
+        for (uint32_t k = 0; k < XCDS; ++k) {
+            const uint32_t x = (b + k) % XCDS;
+            if (head[x] < q[x].size()) {
+                ord.push_back(q[x][head[x]++]);
+                break;
+            }
+        }
+    }
+    PTI_HIP(hipMemcpyAsync(order.ptr, ord.data(), (size_t)nunits * 4, hipMemcpyHostToDevice, s), M);
     // The host vector must outlive the copy.
     PTI_HIP(hipStreamSynchronize(s), M);
     return 0;

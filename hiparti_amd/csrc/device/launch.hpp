@@ -60,7 +60,22 @@ struct RowUnits {
 };
 int build_row_units(const uint32_t *rows, uint64_t n, uint32_t sb, RowUnits &u, hipStream_t s, const char *M);
 int build_unit_order(const uint32_t *unit_chunk, uint32_t nunits, Buffer &order, hipStream_t s, const char *M);
-int build_rank_unit_order(const RowUnits &u, Buffer &order, hipStream_t s, const char *M);
+int build_rank_unit_order(const uint32_t *unit_br, uint32_t nunits, Buffer &order, hipStream_t s, const char *M);
+
+// Windowed entry order of a stream form: inside each block-row the nonzeros
+// are ordered by windows of 2^WINDOW_BITS rows of the largest gathered mode
+// (512 KB of an FP32 factor at rank 32), then by row, and units are dealt by
+// their rank inside their block-row (build_rank_unit_order), so the units in
+// flight on one XCD gather that factor from a few windows held in its L2.
+// Used where block-rows hold two or more units' worth of nonzeros on average
+// and rows keep at least 32 nonzeros per window on average (shorter runs
+// would multiply the run-sum flushes into the LDS tile).
+constexpr uint32_t WINDOW_BITS = 12;
+inline bool windowed_order(uint64_t nnz, uint32_t rows, uint32_t sec_rows, uint32_t row_bits) {
+    const uint64_t blocks = ((uint64_t)rows + (1u << row_bits) - 1) >> row_bits;
+    const uint64_t windows = ((uint64_t)sec_rows + (1u << WINDOW_BITS) - 1) >> WINDOW_BITS;
+    return nnz >= blocks * 2 * UNIT_CHUNKS * 256 && windows >= 4 && nnz >= 32 * (uint64_t)rows * windows;
+}
 
 // Shapes the stream kernel runs: orders up to 5 (beyond, the entry planes and
 // the gathered rows no longer fit the register budget) and lane groups of at
