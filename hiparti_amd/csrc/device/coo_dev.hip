@@ -131,6 +131,13 @@ __global__ void __launch_bounds__(256) window_keys_kernel(const uint32_t *__rest
     }
 }
 
+// Windowed order: nonzeros per window of a mode's coordinates x[0..n).
+__global__ void __launch_bounds__(256) window_count_kernel(const uint32_t *__restrict__ x, uint64_t n,
+                                                           uint32_t *__restrict__ counts) {
+    for (uint64_t z = (uint64_t)blockIdx.x * 256 + threadIdx.x; z < n; z += (uint64_t)gridDim.x * 256)
+        atomicAdd(counts + (x[z] >> WINDOW_BITS), 1u);
+}
+
 inline dim3 grid_1d(uint64_t n) {
     uint64_t b = (n + 255) / 256;
     if (b > 65536) b = 65536;
@@ -245,7 +252,17 @@ int ensure_coo_stream(ptiDevCOO *t, ptiIndex mode, uint32_t ng, hipStream_t s, c
     for (uint32_t m = 0; m < N; ++m)
         if (m != mode && t->ndims[m] > t->ndims[sec]) sec = m;
     const uint64_t nblocks = ((uint64_t)t->ndims[mode] + (1u << ROW_BITS) - 1) >> ROW_BITS;
-    const bool windowed = windowed_order(t->nnz, t->ndims[mode], t->ndims[sec], ROW_BITS);
+    bool windowed = windowed_order(t->nnz, t->ndims[mode], t->ndims[sec], ROW_BITS);
+    if (windowed) {
+        const uint32_t nwin = (uint32_t)(((uint64_t)t->ndims[sec] + (1u << WINDOW_BITS) - 1) >> WINDOW_BITS);
+        Buffer wc;
+        PTI_HIP(wc.alloc((size_t)nwin * 4), M);
+        PTI_HIP(zero_floats(wc.as<float>(), nwin, s), M);   // all-zero bits: count 0
+        hipLaunchKernelGGL(window_count_kernel, grid_1d(t->nnz), dim3(256), 0, s, t->inds[sec].as<uint32_t>(), t->nnz,
+                           wc.as<uint32_t>());
+        PTI_HIP(hipGetLastError(), M);
+        PTI_CHECK(windows_even(wc.as<uint32_t>(), nwin, t->nnz, &windowed, s, M), M);
+    }
     Buffer key, pos, skey, spos, tmp;
     if (windowed) {
         const uint64_t n = t->nnz;

@@ -345,6 +345,24 @@ __global__ void __launch_bounds__(256) stream_keys_kernel(HiCOOArgs h, uint32_t 
     }
 }
 
+// Windowed order: nonzeros of the mode copy per window of mode m's
+// coordinate (counts zeroed by the caller); one wave per chunk.
+__global__ void __launch_bounds__(256) window_hist_kernel(HiCOOArgs h, uint32_t nchunks, uint32_t N, uint32_t m,
+                                                          uint32_t *__restrict__ counts) {
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t c = blockIdx.x * 4 + wid;
+    if (c >= nchunks) return;   // whole wave
+    const uint32_t EW = (N + 3) / 4;
+    const uint32_t z0 = h.chunk_z[c], cnt = h.chunk_z[c + 1] - z0, b0 = h.chunk_blk[c];
+    for (uint32_t zl = lane; zl < cnt; zl += 64) {
+        const uint32_t z = z0 + zl;
+        const uint32_t b = b0 + h.lblk[z];
+        const uint32_t x = (h.binds[(uint64_t)b * N + m] << h.sb_bits) +
+                           ((h.einds[(uint64_t)z * EW + m / 4] >> ((m % 4) * 8)) & 0xFFu);
+        atomicAdd(counts + (x >> WINDOW_BITS), 1u);
+    }
+}
+
 // Step 2, after a stable sort of the nonzeros by (output row, coordinate of
 // the largest non-target mode): each block-row keeps its place, since
 // block-rows are in ascending block coordinate, so the chunk boundaries are
@@ -642,7 +660,17 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
     for (uint32_t m = 0; m < N; ++m)
         if (m != mode && t->ndims[m] >= t->ndims[sec]) sec = m;
     const uint64_t nnz = t->nnz;
-    const bool windowed = windowed_order(nnz, t->ndims[mode], t->ndims[sec], t->sb_bits);
+    bool windowed = windowed_order(nnz, t->ndims[mode], t->ndims[sec], t->sb_bits);
+    if (windowed) {
+        const uint32_t nwin = (uint32_t)(((uint64_t)t->ndims[sec] + (1u << WINDOW_BITS) - 1) >> WINDOW_BITS);
+        Buffer wc;
+        PTI_HIP(wc.alloc((size_t)nwin * 4), M);
+        PTI_HIP(zero_floats(wc.as<float>(), nwin, s), M);   // all-zero bits: count 0
+        hipLaunchKernelGGL(window_hist_kernel, dim3((c.nchunks + 3) / 4), dim3(256), 0, s, hicoo_args(t, mode),
+                           c.nchunks, N, sec, wc.as<uint32_t>());
+        PTI_HIP(hipGetLastError(), M);
+        PTI_CHECK(windows_even(wc.as<uint32_t>(), nwin, nnz, &windowed, s, M), M);
+    }
     Buffer key, pos, skey, spos, blk_of, tmp;
     PTI_HIP(key.alloc(nnz * 8), M);
     PTI_HIP(pos.alloc(nnz * 4), M);
@@ -810,6 +838,16 @@ int build_unit_order(const uint32_t *unit_chunk, uint32_t nunits, Buffer &order,
             M);
     // The temporaries are released on return.
     PTI_HIP(hipStreamSynchronize(s), M);
+    return 0;
+}
+
+int windows_even(const uint32_t *counts, uint32_t nwin, uint64_t nnz, bool *even, hipStream_t s, const char *M) {
+    std::vector<uint32_t> h(nwin);
+    PTI_HIP(hipMemcpyAsync(h.data(), counts, (size_t)nwin * 4, hipMemcpyDeviceToHost, s), M);
+    PTI_HIP(hipStreamSynchronize(s), M);
+    uint32_t mx = 0;
+    for (uint32_t x : h) mx = std::max(mx, x);
+    *even = nwin > 0 && (double)mx <= 4.0 * (double)nnz / (double)nwin;
     return 0;
 }
 

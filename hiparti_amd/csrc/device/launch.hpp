@@ -67,15 +67,23 @@ int build_rank_unit_order(const uint32_t *unit_br, uint32_t nunits, Buffer &orde
 // (512 KB of an FP32 factor at rank 32), then by row, and units are dealt by
 // their rank inside their block-row (build_rank_unit_order), so the units in
 // flight on one XCD gather that factor from a few windows held in its L2.
-// Used where block-rows hold two or more units' worth of nonzeros on average
-// and rows keep at least 32 nonzeros per window on average (shorter runs
-// would multiply the run-sum flushes into the LDS tile).
+// Used where block-rows hold two or more units' worth of nonzeros on average,
+// rows keep at least 32 nonzeros per window on average (shorter runs would
+// multiply the run-sum flushes into the LDS tile) and the gathered mode's
+// coordinates are spread evenly over the windows (windows_even): with a
+// skewed (Zipf) mode most (row, window) runs are far shorter than the
+// average, and its frequent rows stay in L2 anyway.
 constexpr uint32_t WINDOW_BITS = 12;
 inline bool windowed_order(uint64_t nnz, uint32_t rows, uint32_t sec_rows, uint32_t row_bits) {
     const uint64_t blocks = ((uint64_t)rows + (1u << row_bits) - 1) >> row_bits;
     const uint64_t windows = ((uint64_t)sec_rows + (1u << WINDOW_BITS) - 1) >> WINDOW_BITS;
     return nnz >= blocks * 2 * UNIT_CHUNKS * 256 && windows >= 4 && nnz >= 32 * (uint64_t)rows * windows;
 }
+
+// Whether the per-window counts of a gathered mode's coordinates (counts[w]
+// for the nwin windows, in device memory) are even: the largest at most four
+// times the mean.  *even is written on success.
+int windows_even(const uint32_t *counts, uint32_t nwin, uint64_t nnz, bool *even, hipStream_t s, const char *M);
 
 // Shapes the stream kernel runs: orders up to 5 (beyond, the entry planes and
 // the gathered rows no longer fit the register budget) and lane groups of at
