@@ -3,8 +3,9 @@
 //
 // Kernels:
 //   gram_partial / gram_reduce   G = A^T A in 32 x 32 tiles of the upper
-//                                triangle, rows split over workgroups,
-//                                partial tiles summed in split order.
+//                                triangle (one 16 x 16 tile for R <= 16),
+//                                rows split over workgroups, partial tiles
+//                                summed in split order.
 //   solve_normals                one workgroup: V = Hadamard of the other
 //                                modes' Grams, Cholesky (right-looking), else
 //                                LU with partial pivoting (the reference's
@@ -80,6 +81,45 @@ __global__ void __launch_bounds__(256) gram_partial_kernel(const float *__restri
     double *out = partials + ((uint64_t)blockIdx.y * nsplit + blockIdx.x) * (GT * GT);
 #pragma unroll
     for (int k = 0; k < 4; ++k) out[ty * GT + 4 * tx + k] = acc[k];
+}
+
+// Gram partials for R <= 16: the single tile holds at most 16 x 16 entries,
+// so the 256 threads form 4 row phases x 64 entry groups (entry row p = e / 4,
+// columns 4 (e % 4) .. 4 (e % 4) + 3), each phase taking every fourth staged
+// row; the phases are added in a fixed order before the partial is written.
+// Entry (p, q) goes where gram_reduce_kernel reads it (p * GT + q of tile 0);
+// entries past the rank are never read there.
+__global__ void __launch_bounds__(256) gram16_partial_kernel(const float *__restrict__ A, uint64_t nrows, uint32_t R,
+                                                             uint32_t stride, uint64_t rows_per_split,
+                                                             double *__restrict__ partials) {
+    constexpr int E = 16;
+    __shared__ float s[GROWS][E + 1];
+    __shared__ double red[4][E * E];
+    const int tid = threadIdx.x, ph = tid >> 6, e = tid & 63;
+    const int p = e >> 2, q0 = (e & 3) * 4;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const uint64_t i0 = (uint64_t)blockIdx.x * rows_per_split;
+    const uint64_t i1 = i0 + rows_per_split < nrows ? i0 + rows_per_split : nrows;
+    for (uint64_t base = i0; base < i1; base += GROWS) {
+        for (int x = tid; x < GROWS * E; x += 256) {
+            const int r = x / E, c = x % E;
+            const uint64_t i = base + r;
+            s[r][c] = (i < i1 && (uint32_t)c < R) ? A[i * stride + c] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int r = ph; r < GROWS; r += 4) {
+            const double a = s[r][p];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[k] = fma(a, (double)s[r][q0 + k], acc[k]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) red[ph][4 * e + k] = acc[k];   // entry p * 16 + q0 + k == 4 * e + k
+    __syncthreads();
+    const double v = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+    partials[(uint64_t)blockIdx.x * (GT * GT) + (tid / E) * GT + (tid % E)] = v;
 }
 
 // Sums the nsplit partial tiles: workgroup (tile, y) takes the tile's 64
@@ -487,8 +527,12 @@ int dense_gram(const float *A, uint64_t nrows, uint32_t R, uint32_t stride, doub
     if (nsplit == 0) nsplit = 1;
     uint64_t per = (nrows + nsplit - 1) / nsplit;
     per = (per + GROWS - 1) / GROWS * GROWS;
-    hipLaunchKernelGGL(gram_partial_kernel, dim3(nsplit, ntiles), dim3(256), 0, s, A, nrows, R, stride, per, nsplit,
-                       w.partials.as<double>());
+    if (R <= 16)
+        hipLaunchKernelGGL(gram16_partial_kernel, dim3(nsplit), dim3(256), 0, s, A, nrows, R, stride, per,
+                           w.partials.as<double>());
+    else
+        hipLaunchKernelGGL(gram_partial_kernel, dim3(nsplit, ntiles), dim3(256), 0, s, A, nrows, R, stride, per,
+                           nsplit, w.partials.as<double>());
     PTI_HIP(hipGetLastError(), M);
     hipLaunchKernelGGL(gram_reduce_kernel, dim3(ntiles, GT * GT / 64), dim3(256), 0, s, w.partials.as<double>(),
                        nsplit, R, G);

@@ -44,8 +44,9 @@ def cuda(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
-@pytest.mark.parametrize("nrows,R,stride", [(0, 8, 8), (1, 5, 8), (1000, 1, 8), (5000, 16, 16), (4097, 32, 40),
-                                            (20000, 33, 40), (3000, 64, 64), (777, 100, 104), (300, 256, 256)])
+@pytest.mark.parametrize("nrows,R,stride", [(0, 8, 8), (1, 5, 8), (1000, 1, 8), (5000, 16, 16), (70001, 16, 16),
+                                            (3000, 10, 16), (4097, 32, 40), (20000, 33, 40), (3000, 64, 64),
+                                            (777, 100, 104), (300, 256, 256)])
 def test_gram_matches_fp64(nrows, R, stride):
     rng = np.random.default_rng(nrows + R)
     a = np.zeros((nrows, stride), np.float32)
