@@ -12,7 +12,8 @@
 //                                spotrf -> sgesv fallback), then W = V^-1 one
 //                                column per thread.  Exactly singular V leaves
 //                                the right-hand side unchanged, as gesv does.
-//   apply                        A = M W, 64 x 64 output tiles, FP64 sums.
+//   apply                        A = M W, 64 x 64 output tiles (64 x 16 for
+//                                R <= 16), FP64 sums.
 //   colstat_partial / _reduce    column 2-norm, max (clamped >= 1) or dot.
 //   scale                        A[:, r] /= lambda[r].
 //   fit                          ||K||^2 from the full Grams, <X,K>, fit.
@@ -279,40 +280,45 @@ __global__ void __launch_bounds__(256) solve_normals_kernel(const double *__rest
     if (tid == 0) *status = 1;
 }
 
-// A = M W for an nrows x R block of rows (64 x 64 output tiles, 16-deep k
-// slices staged in LDS).  Thread (ty, tx) owns rows ty + 16a, columns
-// tx + 16b, a, b in 0..3.  When status == 2, A = M.
+// A = M W for an nrows x R block of rows (64 x 16 NB output tiles, 16-deep
+// k slices staged in LDS; NB = 4, or 1 for R <= 16).  Thread (ty, tx) owns
+// rows ty + 16a, a in 0..3, and columns tx + 16b, b < NB.  When status == 2,
+// A = M.
+template <int NB>
 __global__ void __launch_bounds__(256) apply_kernel(const float *__restrict__ Mrhs, float *__restrict__ A,
                                                     const double *__restrict__ W, const int *__restrict__ status,
                                                     uint64_t nrows, uint32_t R, uint32_t stride) {
+    constexpr int TC = 16 * NB;   // output columns per tile
     __shared__ float sm[16][64 + 1];
-    __shared__ double sw[16][64];
+    __shared__ double sw[16][TC];
     const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
     const uint64_t r0 = (uint64_t)blockIdx.x * 64;
-    const uint32_t c0 = blockIdx.y * 64;
+    const uint32_t c0 = blockIdx.y * TC;
     const bool copy = *status == 2;
-    double acc[4][4] = {};
+    double acc[4][NB] = {};
     if (!copy) {
         for (uint32_t k0 = 0; k0 < R; k0 += 16) {
             for (int e = tid; e < 64 * 16; e += 256) {
                 const int r = e / 16, k = e % 16;
                 const uint64_t i = r0 + r;
                 sm[k][r] = (i < nrows && k0 + k < R) ? Mrhs[i * stride + k0 + k] : 0.f;
-                const int kk = e / 64, c = e % 64;
+            }
+            for (int e = tid; e < TC * 16; e += 256) {
+                const int kk = e / TC, c = e % TC;
                 sw[kk][c] = (k0 + kk < R && c0 + c < R) ? W[(size_t)(k0 + kk) * R + c0 + c] : 0.0;
             }
             __syncthreads();
 #pragma unroll 4
             for (int k = 0; k < 16; ++k) {
-                double m[4], w[4];
+                double m[4], w[NB];
 #pragma unroll
                 for (int a = 0; a < 4; ++a) m[a] = sm[k][ty + 16 * a];
 #pragma unroll
-                for (int b = 0; b < 4; ++b) w[b] = sw[k][tx + 16 * b];
+                for (int b = 0; b < NB; ++b) w[b] = sw[k][tx + 16 * b];
 #pragma unroll
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
-                    for (int b = 0; b < 4; ++b) acc[a][b] = fma(m[a], w[b], acc[a][b]);
+                    for (int b = 0; b < NB; ++b) acc[a][b] = fma(m[a], w[b], acc[a][b]);
             }
             __syncthreads();
         }
@@ -322,7 +328,7 @@ __global__ void __launch_bounds__(256) apply_kernel(const float *__restrict__ Mr
         const uint64_t i = r0 + ty + 16 * a;
         if (i >= nrows) continue;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
+        for (int b = 0; b < NB; ++b) {
             const uint32_t c = c0 + tx + 16 * b;
             if (c < R) A[i * stride + c] = copy ? Mrhs[i * stride + c] : (float)acc[a][b];
         }
@@ -549,8 +555,12 @@ int dense_solve_normals(const double *grams, uint32_t nmodes, uint32_t mode, uin
                        w.W.as<double>(), w.piv.as<int>(), w.status.as<int>());
     PTI_HIP(hipGetLastError(), M);
     if (nrows == 0) return 0;
-    hipLaunchKernelGGL(apply_kernel, dim3(cdiv(nrows, 64), cdiv(R, 64)), dim3(256), 0, s, Mrhs, A,
-                       w.W.as<double>(), w.status.as<int>(), nrows, R, stride);
+    if (R <= 16)
+        hipLaunchKernelGGL(apply_kernel<1>, dim3(cdiv(nrows, 64), 1), dim3(256), 0, s, Mrhs, A, w.W.as<double>(),
+                           w.status.as<int>(), nrows, R, stride);
+    else
+        hipLaunchKernelGGL(apply_kernel<4>, dim3(cdiv(nrows, 64), cdiv(R, 64)), dim3(256), 0, s, Mrhs, A,
+                           w.W.as<double>(), w.status.as<int>(), nrows, R, stride);
     PTI_HIP(hipGetLastError(), M);
     return 0;
 }

@@ -80,9 +80,9 @@ def test_solve_normals_cholesky(R, mode):
     assert rel_fro(X.cpu().numpy(), ref) <= 1e-6
 
 
-def test_solve_normals_lu_and_singular():
+@pytest.mark.parametrize("R", [24, 12])
+def test_solve_normals_lu_and_singular(R):
     rng = np.random.default_rng(3)
-    R = 24
     S = rng.standard_normal((3, R, R))
     S = S + S.transpose(0, 2, 1)            # symmetric, indefinite Hadamard product
     M = rng.uniform(-1, 1, (500, R)).astype(np.float32)
