@@ -816,10 +816,16 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
 // at four to six waves per SIMD were within 1% of each other, the kernel
 // being limited by the vector-memory path's throughput rather than by
 // latency; D = 4 keeps the smallest register budget.
+// Up to 12 registers of gathered rows per position (order 4 at four FP32 or
+// eight BF16 ranks per lane) still fit D = 4 without spills at four waves per
+// SIMD (104-116 VGPRs); measured 3.5% faster than D = 2 on the 200M-nonzero
+// order-4 tensor at R = 64.  The CP-completion gradient walk, which also
+// holds the target row's sums, keeps D_RESID (D = 4 spills there).
 template <int N, int V, typename FT>
 struct StreamConfig {
     static constexpr int ROW_REGS = (N - 1) * V * (int)sizeof(FT) / 4;
-    static constexpr int D = ROW_REGS <= 8 ? 4 : (ROW_REGS <= 16 ? 2 : 1);
+    static constexpr int D = ROW_REGS <= 12 ? 4 : (ROW_REGS <= 16 ? 2 : 1);
+    static constexpr int D_RESID = ROW_REGS <= 8 ? 4 : (ROW_REGS <= 16 ? 2 : 1);
     static constexpr int OCC = N > 3 ? 4 : ((int)sizeof(FT) * V >= 16 ? 5 : 6);
 };
 
@@ -881,7 +887,7 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, FT>::OCC))
 template <int N, int V, int G, int W>
 __global__ void __launch_bounds__(W * 64, 4)
     completion_grad_kernel(StreamArgs h, FactorArgs a, double *loss_part) {
-    constexpr int D = StreamConfig<N, V, float>::D;
+    constexpr int D = StreamConfig<N, V, float>::D_RESID;
     constexpr int NP = StreamShape<N>::NP;
     extern __shared__ float tile[];  // (1 << sb_bits) rows x TP floats, then as many rows x TRP
     constexpr int GV = G * V;
