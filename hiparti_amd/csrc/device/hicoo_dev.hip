@@ -712,6 +712,42 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
 }
 }  // namespace
 
+// LDS-resident factors for the stream kernel (orders 4 and 5, four or eight
+// ranks per lane): the last gathered modes, in ascending mode order, whose
+// whole rank-tile copies (rows x G * V elements) fit beside the LDS tile in
+// the 80 KB a workgroup may take at two workgroups (four waves per SIMD) per
+// CU, at most one for order 4 and two for order 5 (the instantiated kernels),
+// and only where a unit holds at least four
+// nonzeros per resident row on average (each workgroup loads the copies
+// once).  rows[j] receives the row counts, 0 where none.
+void resident_factors(const ptiDevHiCOO *t, uint32_t mode, Shape sh, size_t esz, uint32_t nunits, uint32_t rows[2]) {
+    constexpr size_t BUDGET = 80 * 1024;
+    rows[0] = rows[1] = 0;
+    const uint32_t N = t->nmodes;
+    if ((N != 4 && N != 5) || sh.V < 4 || nunits == 0) return;
+    const size_t rowb = (size_t)sh.G * sh.V * esz;
+    size_t used = res_lds_offset(1u << t->sb_bits, sh.G * sh.V + 1);
+    uint32_t take[2] = {0, 0}, nres = 0;
+    const uint32_t max_res = N == 4 ? 1 : 2;
+    for (int i = (int)N - 2; i >= 0 && nres < max_res; --i) {
+        const uint32_t m = (uint32_t)i < mode ? (uint32_t)i : (uint32_t)i + 1;
+        const size_t bytes = (size_t)t->ndims[m] * rowb;
+        if (used + bytes > BUDGET) break;
+        used += bytes;
+        take[nres++] = t->ndims[m];
+    }
+    uint64_t total = (uint64_t)take[0] + take[1];
+    if (nres == 0 || 4 * total > t->nnz / nunits) return;
+    // Positions K - nres .. K - 1 in ascending order: the last one found
+    // first is the highest position.
+    if (nres == 1) {
+        rows[0] = take[0];
+    } else {
+        rows[0] = take[1];
+        rows[1] = take[0];
+    }
+}
+
 // Shared by the FP32 and BF16 entry points: zero the output rows, then run the
 // stream kernel (factor matrices < 4 GiB) or the tile kernel once per rank
 // column tile.
@@ -797,6 +833,7 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
             h.nrows = t->ndims[mode];
             h.coord24 = coords_fit_u24(t->ndims, N, mode, stride * esz);
             h.unit_order = c.unit_order.as<uint32_t>();
+            resident_factors(t, mode, sh, esz, c.nunits, h.res_rows);
             PTI_HIP(launch_stream(N, h, a, sh, bf16, s), M);
         } else {
             PTI_HIP(launch_hicoo(N, hicoo_args(t, mode), a, sh, bf16, s), M);

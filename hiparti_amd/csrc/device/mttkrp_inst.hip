@@ -57,13 +57,44 @@ hipError_t hicoo_vg(const HiCOOArgs &h, const FactorArgs &a, bool bf16, hipStrea
     }
 }
 
+// The stream kernel with NRES LDS-resident factors (orders 4 and 5, four or
+// eight ranks per lane); dynamic LDS beyond 64 KB is allowed explicitly.
+template <int V, int G, typename FT, int NRES>
+hipError_t stream_res(const StreamArgs &h, const FactorArgs &a, size_t lds, hipStream_t s) {
+    const void *fn = reinterpret_cast<const void *>(&hicoo_stream_kernel<N, V, G, FT, 8, NRES>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((hicoo_stream_kernel<N, V, G, FT, 8, NRES>), dim3(h.nunits), dim3(512), lds, s, h, a);
+    return hipGetLastError();
+}
+
 template <int V, int G>
 hipError_t stream_vg(const StreamArgs &h, const FactorArgs &a, bool bf16, hipStream_t s) {
     if constexpr (G * V > 64 || !stream_supported(N, G)) {
         return hipErrorInvalidValue;  // the LDS tile holds at most 64 ranks
     } else {
         if (h.nunits == 0) return hipSuccess;
-        const size_t lds = (size_t)(1u << h.sb_bits) * (G * V + 1) * sizeof(float);
+        const uint32_t rows = 1u << h.sb_bits;
+        const int nres = h.res_rows[0] ? (h.res_rows[1] ? 2 : 1) : 0;
+        if (nres) {
+            if constexpr ((N == 4 || N == 5) && V >= 4) {
+                const size_t esz = bf16 ? 2 : 4;
+                const size_t lds = res_lds_offset(rows, G * V + 1) +
+                                   (size_t)(h.res_rows[0] + h.res_rows[1]) * G * V * esz;
+                // Order 4 takes one resident factor, order 5 up to two.
+                if (N == 4 && nres > 1) return hipErrorInvalidValue;
+                if (bf16) {
+                    if (nres == 1) return stream_res<V, G, uint16_t, 1>(h, a, lds, s);
+                    if constexpr (N == 5) return stream_res<V, G, uint16_t, 2>(h, a, lds, s);
+                }
+                if constexpr (V == 4) {
+                    if (nres == 1) return stream_res<V, G, float, 1>(h, a, lds, s);
+                    if constexpr (N == 5) return stream_res<V, G, float, 2>(h, a, lds, s);
+                }
+            }
+            return hipErrorInvalidValue;
+        }
+        const size_t lds = (size_t)rows * (G * V + 1) * sizeof(float);
         dim3 grid(h.nunits);
         if constexpr (V == 8) {
             // Eight ranks per lane exist only for BF16 factors (16-byte pieces).

@@ -587,7 +587,15 @@ struct StreamArgs {
     // unit_order[b], so the hardware's round-robin dealing gives every XCD
     // an even share and the small units run last.  Null: remap_block order.
     const uint32_t *unit_order;
+    // LDS-resident factors (kernels instantiated with NRES > 0): the rows of
+    // the last NRES gathered modes, which the workgroup copies whole into LDS.
+    uint32_t res_rows[2];
 };
+
+// Byte offset of the LDS-resident factors behind a tile of `rows` x tp floats.
+__host__ __device__ constexpr uint32_t res_lds_offset(uint32_t rows, uint32_t tp) {
+    return (rows * tp * 4u + 15u) & ~15u;
+}
 
 __device__ __forceinline__ uint32_t stream_unit(const StreamArgs &h) {
     return h.unit_order ? h.unit_order[blockIdx.x] : remap_block(blockIdx.x, gridDim.x);
@@ -690,10 +698,15 @@ struct TileSink {
 // residual e = x - sum_r prod_m A_m[i_m][r], the target row's factor row
 // read from trow (the block-row's rows of A_mode in LDS, pitch TRP floats,
 // rows past the block-row clamped to trow_last); dummies contribute 0.
-template <int N, int V, int G, int D, typename FT, bool U24, bool RESID = false, typename Sink>
+// NRES > 0: the last NRES gathered factors (ascending mode order) are read
+// from their whole copies in LDS at res (G * V elements per row, rows of the
+// first resident factor, then of the second), every lane of every position,
+// so the walk stays free of branches.
+template <int N, int V, int G, int D, typename FT, bool U24, bool RESID = false, int NRES = 0, typename Sink>
 __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__restrict__ ent, const Sink &sink,
                                             int lane, const float *trow = nullptr, uint32_t trow_last = 0,
-                                            double *esq_sum = nullptr) {
+                                            double *esq_sum = nullptr, const char *res = nullptr,
+                                            uint32_t res1_off = 0) {
     // Sum of squared residuals (RESID with esq_sum): kept in a register and
     // added to *esq_sum once at the end.
     double esq = 0.0;
@@ -728,11 +741,18 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
     VT raw[D][K];
     float vv[D];
     uint32_t rr[D];
+    constexpr uint32_t RRB = G * V * (uint32_t)sizeof(FT);   // bytes per resident row in LDS
     auto issue = [&](auto P) {
         constexpr int p = decltype(P)::value;
 #pragma unroll
-        for (int i = 0; i < K; ++i)
-            raw[p % D][i] = *reinterpret_cast<const VT *>(f[i] + (uint64_t)row_offset<U24>(word(P, i), rowb, colb));
+        for (int i = 0; i < K; ++i) {
+            if (i >= K - NRES) {
+                const uint32_t off = (i == K - NRES ? 0u : res1_off) + word(P, i) * RRB + colb;
+                raw[p % D][i] = *static_cast<const VT *>(__builtin_assume_aligned(res + off, sizeof(VT)));
+            } else {
+                raw[p % D][i] = *reinterpret_cast<const VT *>(f[i] + (uint64_t)row_offset<U24>(word(P, i), rowb, colb));
+            }
+        }
         vv[p % D] = __uint_as_float(word(P, K));
         rr[p % D] = word(P, K + 1);
     };
@@ -834,12 +854,15 @@ struct StreamConfig {
 // back once (stores when the unit owns the block-row, else atomics).  The
 // factor gathers address a uniform base with 32-bit byte offsets: the host
 // sends factor matrices of 4 GiB or more to the tile kernel.
-template <int N, int V, int G, typename FT, int W>
+// NRES > 0: the workgroup first copies the last NRES gathered factors whole
+// (h.res_rows rows each, the rank tile's G * V columns, zero past the rank)
+// into LDS behind the tile; the walk then reads them there.
+template <int N, int V, int G, typename FT, int W, int NRES = 0>
 __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, FT>::OCC))
     hicoo_stream_kernel(StreamArgs h, FactorArgs a) {
     constexpr int D = StreamConfig<N, V, FT>::D;
     constexpr int NP = StreamShape<N>::NP;
-    extern __shared__ float tile[];  // (1 << sb_bits) rows x TP floats
+    extern __shared__ float tile[];  // (1 << sb_bits) rows x TP floats [, resident factors]
     constexpr int GV = G * V;
     constexpr int TP = GV + 1;
     constexpr int NT = W * 64;
@@ -847,15 +870,40 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, FT>::OCC))
     const uint32_t u = stream_unit(h);   // gridDim.x == nunits
     const uint32_t rows = 1u << h.sb_bits;
     for (uint32_t i = tid; i < rows * TP; i += NT) tile[i] = 0.f;
+    char *res = nullptr;
+    uint32_t res1_off = 0;
+    if constexpr (NRES > 0) {
+        using VT = typename Vec<FT, V>::T;
+        constexpr uint32_t RRB = GV * (uint32_t)sizeof(FT);
+        constexpr int K = N - 1;
+        res = reinterpret_cast<char *>(tile) + res_lds_offset(rows, TP);
+        res1_off = h.res_rows[0] * RRB;
+        const uint32_t rowb = a.stride * (uint32_t)sizeof(FT);
+#pragma unroll
+        for (int j = 0; j < NRES; ++j) {
+            const int i = K - NRES + j;   // gathered position, ascending mode order
+            const char *fi = static_cast<const char *>(a.fac[(uint32_t)i < a.mode ? i : i + 1]);
+            char *dst = res + (j == 0 ? 0u : res1_off);
+            const uint32_t n = h.res_rows[j] * G;
+            for (uint32_t x = tid; x < n; x += NT) {
+                const uint32_t r = x / G, v = x % G;
+                VT val{};
+                if (v * V < a.R) val = *reinterpret_cast<const VT *>(fi + (uint64_t)r * rowb + v * V * sizeof(FT));
+                *static_cast<VT *>(__builtin_assume_aligned(dst + r * RRB + v * V * sizeof(FT), sizeof(VT))) = val;
+            }
+        }
+    }
     __syncthreads();
     const uint32_t c0 = h.unit_chunk[u], c1 = h.unit_chunk[u + 1];
     const uint32_t w0 = __builtin_amdgcn_readfirstlane(wid);
     if (h.coord24) {
         for (uint32_t c = c0 + w0; c < c1; c += W)
-            stream_walk<N, V, G, D, FT, true>(a, h.ent + (uint64_t)c * NP * CHUNK, TileSink{tile, TP}, lane);
+            stream_walk<N, V, G, D, FT, true, false, NRES>(a, h.ent + (uint64_t)c * NP * CHUNK, TileSink{tile, TP},
+                                                            lane, nullptr, 0, nullptr, res, res1_off);
     } else {
         for (uint32_t c = c0 + w0; c < c1; c += W)
-            stream_walk<N, V, G, D, FT, false>(a, h.ent + (uint64_t)c * NP * CHUNK, TileSink{tile, TP}, lane);
+            stream_walk<N, V, G, D, FT, false, false, NRES>(a, h.ent + (uint64_t)c * NP * CHUNK, TileSink{tile, TP},
+                                                             lane, nullptr, 0, nullptr, res, res1_off);
     }
     __syncthreads();
     const uint32_t base = h.unit_br[u] << h.sb_bits;
