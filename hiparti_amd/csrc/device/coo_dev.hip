@@ -134,8 +134,11 @@ __global__ void __launch_bounds__(256) window_keys_kernel(const uint32_t *__rest
 // Windowed order: nonzeros per window of a mode's coordinates x[0..n).
 __global__ void __launch_bounds__(256) window_count_kernel(const uint32_t *__restrict__ x, uint64_t n,
                                                            uint32_t *__restrict__ counts) {
-    for (uint64_t z = (uint64_t)blockIdx.x * 256 + threadIdx.x; z < n; z += (uint64_t)gridDim.x * 256)
-        atomicAdd(counts + (x[z] >> WINDOW_BITS), 1u);
+    // Every lane of a wave runs the same iterations (wave_count is wave-wide).
+    for (uint64_t z0 = (uint64_t)blockIdx.x * 256 + (threadIdx.x & ~63u); z0 < n; z0 += (uint64_t)gridDim.x * 256) {
+        const uint64_t z = z0 + (threadIdx.x & 63u);
+        wave_count(counts, z < n ? x[z] >> WINDOW_BITS : 0u, z < n);
+    }
 }
 
 inline dim3 grid_1d(uint64_t n) {

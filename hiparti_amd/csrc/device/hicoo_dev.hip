@@ -354,12 +354,16 @@ __global__ void __launch_bounds__(256) window_hist_kernel(HiCOOArgs h, uint32_t 
     if (c >= nchunks) return;   // whole wave
     const uint32_t EW = (N + 3) / 4;
     const uint32_t z0 = h.chunk_z[c], cnt = h.chunk_z[c + 1] - z0, b0 = h.chunk_blk[c];
-    for (uint32_t zl = lane; zl < cnt; zl += 64) {
-        const uint32_t z = z0 + zl;
-        const uint32_t b = b0 + h.lblk[z];
-        const uint32_t x = (h.binds[(uint64_t)b * N + m] << h.sb_bits) +
-                           ((h.einds[(uint64_t)z * EW + m / 4] >> ((m % 4) * 8)) & 0xFFu);
-        atomicAdd(counts + (x >> WINDOW_BITS), 1u);
+    for (uint32_t zb = 0; zb < cnt; zb += 64) {   // the same iterations in every lane
+        const uint32_t zl = zb + lane;
+        uint32_t x = 0;
+        if (zl < cnt) {
+            const uint32_t z = z0 + zl;
+            const uint32_t b = b0 + h.lblk[z];
+            x = (h.binds[(uint64_t)b * N + m] << h.sb_bits) +
+                ((h.einds[(uint64_t)z * EW + m / 4] >> ((m % 4) * 8)) & 0xFFu);
+        }
+        wave_count(counts, x >> WINDOW_BITS, zl < cnt);
     }
 }
 
@@ -712,8 +716,8 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
 }
 }  // namespace
 
-// LDS-resident factors for the stream kernel (orders 4 and 5, four or eight
-// ranks per lane): the last gathered modes, in ascending mode order, whose
+// LDS-resident factors for the stream kernel (orders 4 and 5, four ranks per
+// lane): the last gathered modes, in ascending mode order, whose
 // whole rank-tile copies (rows x G * V elements) fit beside the LDS tile in
 // the 80 KB a workgroup may take at two workgroups (four waves per SIMD) per
 // CU, at most one for order 4 and two for order 5 (the instantiated kernels),
@@ -724,7 +728,11 @@ void resident_factors(const ptiDevHiCOO *t, uint32_t mode, Shape sh, size_t esz,
     constexpr size_t BUDGET = 80 * 1024;
     rows[0] = rows[1] = 0;
     const uint32_t N = t->nmodes;
-    if ((N != 4 && N != 5) || sh.V < 4 || nunits == 0) return;
+    // Eight BF16 ranks per lane (16-byte lanes) gain nothing from the copies:
+    // on the 200M-nonzero order-4 tensor at R = 64 that kernel is not bound
+    // by its gathers, and the copies took modes 0-2 from 5.57 / 3.48 / 3.39
+    // to 5.75 / 3.53 / 3.50 ms.
+    if ((N != 4 && N != 5) || sh.V != 4 || nunits == 0) return;
     const size_t rowb = (size_t)sh.G * sh.V * esz;
     size_t used = res_lds_offset(1u << t->sb_bits, sh.G * sh.V + 1);
     uint32_t take[2] = {0, 0}, nres = 0;

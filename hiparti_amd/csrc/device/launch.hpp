@@ -85,6 +85,22 @@ inline bool windowed_order(uint64_t nnz, uint32_t rows, uint32_t sec_rows, uint3
 // times the mean.  *even is written on success.
 int windows_even(const uint32_t *counts, uint32_t nwin, uint64_t nnz, bool *even, hipStream_t s, const char *M);
 
+// Histogram update for a whole wave (every lane calls it): adds one to
+// counts[x] for each lane with `valid`, as one atomic per distinct x in the
+// wave, so that skewed (Zipf) coordinates do not queue millions of atomics
+// on one counter.
+__device__ __forceinline__ void wave_count(uint32_t *counts, uint32_t x, bool valid) {
+    uint64_t left = __builtin_amdgcn_ballot_w64(valid);
+    while (left) {
+        const int leader = __builtin_ctzll(left);
+        const uint32_t xl = (uint32_t)__builtin_amdgcn_readlane((int)x, leader);
+        const uint64_t same = __builtin_amdgcn_ballot_w64(valid && x == xl);
+        if ((int)(threadIdx.x & 63) == leader) atomicAdd(counts + xl, (uint32_t)__builtin_popcountll(same));
+        valid = valid && x != xl;
+        left &= ~same;
+    }
+}
+
 // Shapes the stream kernel runs: orders up to 5 (beyond, the entry planes and
 // the gathered rows no longer fit the register budget) and lane groups of at
 // most 16 lanes (a slice of at most 64 positions, walked straight-line).

@@ -57,8 +57,8 @@ hipError_t hicoo_vg(const HiCOOArgs &h, const FactorArgs &a, bool bf16, hipStrea
     }
 }
 
-// The stream kernel with NRES LDS-resident factors (orders 4 and 5, four or
-// eight ranks per lane); dynamic LDS beyond 64 KB is allowed explicitly.
+// The stream kernel with NRES LDS-resident factors (orders 4 and 5, four
+// ranks per lane); dynamic LDS beyond 64 KB is allowed explicitly.
 template <int V, int G, typename FT, int NRES>
 hipError_t stream_res(const StreamArgs &h, const FactorArgs &a, size_t lds, hipStream_t s) {
     const void *fn = reinterpret_cast<const void *>(&hicoo_stream_kernel<N, V, G, FT, 8, NRES>);
@@ -77,7 +77,7 @@ hipError_t stream_vg(const StreamArgs &h, const FactorArgs &a, bool bf16, hipStr
         const uint32_t rows = 1u << h.sb_bits;
         const int nres = h.res_rows[0] ? (h.res_rows[1] ? 2 : 1) : 0;
         if (nres) {
-            if constexpr ((N == 4 || N == 5) && V >= 4) {
+            if constexpr ((N == 4 || N == 5) && V == 4) {
                 const size_t esz = bf16 ? 2 : 4;
                 const size_t lds = res_lds_offset(rows, G * V + 1) +
                                    (size_t)(h.res_rows[0] + h.res_rows[1]) * G * V * esz;
@@ -87,10 +87,8 @@ hipError_t stream_vg(const StreamArgs &h, const FactorArgs &a, bool bf16, hipStr
                     if (nres == 1) return stream_res<V, G, uint16_t, 1>(h, a, lds, s);
                     if constexpr (N == 5) return stream_res<V, G, uint16_t, 2>(h, a, lds, s);
                 }
-                if constexpr (V == 4) {
-                    if (nres == 1) return stream_res<V, G, float, 1>(h, a, lds, s);
-                    if constexpr (N == 5) return stream_res<V, G, float, 2>(h, a, lds, s);
-                }
+                if (nres == 1) return stream_res<V, G, float, 1>(h, a, lds, s);
+                if constexpr (N == 5) return stream_res<V, G, float, 2>(h, a, lds, s);
             }
             return hipErrorInvalidValue;
         }

@@ -1,6 +1,6 @@
 """GPU checks of the stream kernel with LDS-resident factors.
 
-For orders 4 and 5 at four (or, BF16, eight) ranks per lane, the last
+For orders 4 and 5 at four ranks per lane (FP32 and BF16), the last
 gathered modes in ascending mode order whose whole rank-tile copies fit
 beside the LDS tile in 80 KB are copied into LDS by every workgroup (at most
 one for order 4, two for order 5, and only where a unit holds at least four
@@ -9,7 +9,8 @@ below are shaped so that this happens: short trailing modes and few, long
 block-rows.  For order 5 at R = 16 and mode 0 the two trailing modes (300
 and 24 rows) are resident, for mode 4 only the 300-row one (the 1000-row
 mode would not fit beside it); for order 4 at R = 64 the 100-row mode is
-resident for modes 0-2.  R = 96 runs two column tiles (64 and 32 ranks)
+resident for modes 0-2 with FP32 factors (BF16 factors take eight ranks per
+lane there and no copies).  R = 96 runs two column tiles (64 and 32 ranks)
 that each copy their own columns of the resident factors.  Every output is
 compared with the FP64 oracle (BF16: on the bf16-rounded factors), and the
 same tensors at R = 18 (two ranks per lane, no resident factors) cover the
