@@ -87,7 +87,9 @@ struct Ptr8 {
 // the StreamShape layout for ng lane groups, the coordinates of the other
 // modes in ascending mode order, the value bits and the row inside its block.
 // perm (optional): the entries of each block in another order (positions
-// inside the block's range of the sorted copy).
+// inside the block's range of the sorted copy).  A chunk of at most
+// INTERLEAVE_MAX_RUNS row runs is interleaved: slice g takes its entries g,
+// g + ng, g + 2 ng, ...
 constexpr uint32_t ROW_BITS = 7;
 __global__ void __launch_bounds__(256) coo_stream_fill_kernel(Ptr8 sinds, const float *__restrict__ svals,
                                                               const uint32_t *__restrict__ chunk_z, uint32_t nchunks,
@@ -101,13 +103,22 @@ __global__ void __launch_bounds__(256) coo_stream_fill_kernel(Ptr8 sinds, const 
     const uint32_t z0 = chunk_z[c], cnt = chunk_z[c + 1] - z0;
     const uint32_t per = (cnt + ng - 1) / ng;
     const uint32_t G = 64 / ng;
+    auto row_of = [&](uint64_t zs) { return sinds.p[mode][perm ? perm[zs] : zs]; };
+    uint32_t breaks = 0;   // row changes between neighbouring entries (the same in every lane)
+    for (uint32_t kb = 1; kb < cnt; kb += 64) {
+        const uint32_t k = kb + lane;
+        const bool brk = k < cnt && row_of((uint64_t)z0 + k) != row_of((uint64_t)z0 + k - 1);
+        breaks += (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(brk));
+    }
+    const bool interleave = breaks < INTERLEAVE_MAX_RUNS;
     for (uint32_t e = lane; e < CHUNK; e += 64) {
         const uint32_t g = (e % 64) / G, j = (e / 64) * G + e % G;   // e == stream_entry(G, g, j)
         uint32_t w[12];
         for (uint32_t k = 0; k < 12; ++k) w[k] = 0u;
         w[N] = DUMMY_ROW;
-        if (j < per && g * per + j < cnt) {
-            const uint64_t zs = (uint64_t)z0 + g * per + j;
+        const uint32_t k = interleave ? j * ng + g : g * per + j;
+        if (j < per && k < cnt) {
+            const uint64_t zs = (uint64_t)z0 + k;
             const uint64_t z = perm ? perm[zs] : zs;
             for (uint32_t i = 0; i + 1 < N; ++i) w[i] = sinds.p[i < mode ? i : i + 1][z];
             w[N - 1] = __float_as_uint(svals[z]);
@@ -118,15 +129,22 @@ __global__ void __launch_bounds__(256) coo_stream_fill_kernel(Ptr8 sinds, const 
     }
 }
 
-// Sort keys of the windowed order: (row block, window of the gathered
-// coordinate `sec`, row inside the block), over the copy sorted by `mode`.
-__global__ void __launch_bounds__(256) window_keys_kernel(const uint32_t *__restrict__ rows,
+// Sort keys of a stream form's order over the copy sorted by `mode`, as for
+// the HiCOO stream forms: (row, coordinate of the largest gathered mode
+// `sec`), whose factor rows are the likeliest cache misses; or, windowed
+// (wbits > 0), (row block, window of `sec`, row inside the block, the low
+// WINDOW_BITS of `sec`).
+__global__ void __launch_bounds__(256) stream_keys_kernel(const uint32_t *__restrict__ rows,
                                                           const uint32_t *__restrict__ sec, uint64_t n, uint32_t wbits,
                                                           uint64_t *__restrict__ key, uint32_t *__restrict__ pos) {
     for (uint64_t z = (uint64_t)blockIdx.x * 256 + threadIdx.x; z < n; z += (uint64_t)gridDim.x * 256) {
-        const uint32_t r = rows[z];
-        key[z] = ((uint64_t)(r >> ROW_BITS) << (wbits + ROW_BITS)) | ((uint64_t)(sec[z] >> WINDOW_BITS) << ROW_BITS) |
-                 (r & ((1u << ROW_BITS) - 1u));
+        const uint32_t r = rows[z], x = sec[z];
+        if (wbits)
+            key[z] = ((uint64_t)(r >> ROW_BITS) << (wbits + ROW_BITS + WINDOW_BITS)) |
+                     ((uint64_t)(x >> WINDOW_BITS) << (ROW_BITS + WINDOW_BITS)) |
+                     ((uint64_t)(r & ((1u << ROW_BITS) - 1u)) << WINDOW_BITS) | (x & ((1u << WINDOW_BITS) - 1u));
+        else
+            key[z] = ((uint64_t)r << 32) | x;
         pos[z] = (uint32_t)z;
     }
 }
@@ -267,19 +285,22 @@ int ensure_coo_stream(ptiDevCOO *t, ptiIndex mode, uint32_t ng, hipStream_t s, c
         PTI_CHECK(windows_even(wc.as<uint32_t>(), nwin, t->nnz, &windowed, s, M), M);
     }
     Buffer key, pos, skey, spos, tmp;
-    if (windowed) {
-        const uint64_t n = t->nnz;
-        unsigned bbits = 0, wbits = 0;
+    const uint64_t n = t->nnz;
+    if (n) {
+        unsigned bbits = 0, wbits = 0, rbits = 1;
         while (bbits < 32 && ((nblocks - 1) >> bbits)) ++bbits;
-        while (wbits < 32 && (((uint64_t)t->ndims[sec] - 1) >> WINDOW_BITS) >> wbits) ++wbits;
+        while (rbits < 32 && (((uint64_t)t->ndims[mode] - 1) >> rbits)) ++rbits;
+        if (windowed)
+            while (wbits < 32 && (((uint64_t)t->ndims[sec] - 1) >> WINDOW_BITS) >> wbits) ++wbits;
         PTI_HIP(key.alloc(n * 8), M);
         PTI_HIP(skey.alloc(n * 8), M);
         PTI_HIP(pos.alloc(n * 4), M);
         PTI_HIP(spos.alloc(n * 4), M);
-        hipLaunchKernelGGL(window_keys_kernel, grid_1d(n), dim3(256), 0, s, t->sinds[mode][mode].as<uint32_t>(),
-                           t->sinds[mode][sec].as<uint32_t>(), n, wbits, key.as<uint64_t>(), pos.as<uint32_t>());
+        hipLaunchKernelGGL(stream_keys_kernel, grid_1d(n), dim3(256), 0, s, t->sinds[mode][mode].as<uint32_t>(),
+                           t->sinds[mode][sec].as<uint32_t>(), n, windowed ? wbits : 0u, key.as<uint64_t>(),
+                           pos.as<uint32_t>());
         PTI_HIP(hipGetLastError(), M);
-        const unsigned end_bit = bbits + wbits + ROW_BITS;
+        const unsigned end_bit = windowed ? bbits + wbits + ROW_BITS + WINDOW_BITS : rbits + 32;
         size_t tb = 0;
         PTI_HIP(rocprim::radix_sort_pairs(nullptr, tb, key.as<uint64_t>(), skey.as<uint64_t>(), pos.as<uint32_t>(),
                                           spos.as<uint32_t>(), (size_t)n, 0u, end_bit, s),
@@ -291,7 +312,7 @@ int ensure_coo_stream(ptiDevCOO *t, ptiIndex mode, uint32_t ng, hipStream_t s, c
     }
     hipLaunchKernelGGL(coo_stream_fill_kernel, dim3((u.nchunks + 3) / 4), dim3(256), 0, s, p,
                        t->svals[mode].as<float>(), u.chunk_z.as<uint32_t>(), u.nchunks, N, (uint32_t)mode, ng,
-                       windowed ? spos.as<uint32_t>() : nullptr, t->stream[mode].as<uint4>());
+                       n ? spos.as<uint32_t>() : nullptr, t->stream[mode].as<uint4>());
     PTI_HIP(hipGetLastError(), M);
     if (windowed)
         PTI_CHECK(build_rank_unit_order(u.unit_br.as<uint32_t>(), u.nunits, t->unit_order[mode], s, M), M);

@@ -28,6 +28,7 @@ namespace {
 
 constexpr int GT = 32;          // Gram tile edge
 constexpr int GROWS = 64;       // rows staged per LDS pass in the Gram kernel
+constexpr int G16ROWS = 256;    // rows staged per LDS pass in the R <= 16 Gram kernel
 constexpr uint32_t MAX_PARTS = 1024;  // workgroups contributing partial sums
 
 __host__ __device__ inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -88,28 +89,31 @@ __global__ void __launch_bounds__(256) gram_partial_kernel(const float *__restri
 // so the 256 threads form 4 row phases x 64 entry groups (entry row p = e / 4,
 // columns 4 (e % 4) .. 4 (e % 4) + 3), each phase taking every fourth staged
 // row; the phases are added in a fixed order before the partial is written.
+// Rows are staged G16ROWS at a time (16 independent loads per thread), so a
+// workgroup's 1024 rows wait on four load latencies, not sixteen.
 // Entry (p, q) goes where gram_reduce_kernel reads it (p * GT + q of tile 0);
 // entries past the rank are never read there.
 __global__ void __launch_bounds__(256) gram16_partial_kernel(const float *__restrict__ A, uint64_t nrows, uint32_t R,
                                                              uint32_t stride, uint64_t rows_per_split,
                                                              double *__restrict__ partials) {
     constexpr int E = 16;
-    __shared__ float s[GROWS][E + 1];
+    __shared__ float s[G16ROWS][E + 1];
     __shared__ double red[4][E * E];
     const int tid = threadIdx.x, ph = tid >> 6, e = tid & 63;
     const int p = e >> 2, q0 = (e & 3) * 4;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     const uint64_t i0 = (uint64_t)blockIdx.x * rows_per_split;
     const uint64_t i1 = i0 + rows_per_split < nrows ? i0 + rows_per_split : nrows;
-    for (uint64_t base = i0; base < i1; base += GROWS) {
-        for (int x = tid; x < GROWS * E; x += 256) {
+    for (uint64_t base = i0; base < i1; base += G16ROWS) {
+#pragma unroll
+        for (int x = tid; x < G16ROWS * E; x += 256) {
             const int r = x / E, c = x % E;
             const uint64_t i = base + r;
             s[r][c] = (i < i1 && (uint32_t)c < R) ? A[i * stride + c] : 0.f;
         }
         __syncthreads();
 #pragma unroll 4
-        for (int r = ph; r < GROWS; r += 4) {
+        for (int r = ph; r < G16ROWS; r += 4) {
             const double a = s[r][p];
 #pragma unroll
             for (int k = 0; k < 4; ++k) acc[k] = fma(a, (double)s[r][q0 + k], acc[k]);
@@ -154,14 +158,32 @@ __global__ void __launch_bounds__(256) gram_reduce_kernel(const double *__restri
     }
 }
 
-// One workgroup.  V, W: R x R scratch; piv: R ints; status: 1 int.
+// One workgroup.  Vg, Wg: R x R scratch in device memory; piv: R ints;
+// status: 1 int.  For R <= SOLVE_LDS_RANK the factorization and the inverse
+// run in dynamic LDS (2 R^2 doubles, which the launch provides) and only W is
+// written back: every elimination step is a workgroup barrier, and with the
+// matrices in device memory each step also waits on L2 round trips (37 us
+// per solve at R = 16).
+constexpr uint32_t SOLVE_LDS_RANK = 64;
 __global__ void __launch_bounds__(256) solve_normals_kernel(const double *__restrict__ grams, uint32_t nmodes,
-                                                            uint32_t mode, uint32_t R, double *V, double *W,
+                                                            uint32_t mode, uint32_t R, double *Vg, double *Wg,
                                                             int *piv, int *status) {
     __shared__ int s_state;  // 0 running, 1 not SPD, 2 singular
     __shared__ int s_piv;
+    extern __shared__ double s_mats[];
     const int tid = threadIdx.x;
     const uint32_t RR = R * R;
+    const bool in_lds = R <= SOLVE_LDS_RANK;
+    double *V = in_lds ? s_mats : Vg;
+    double *W = in_lds ? s_mats + RR : Wg;
+    // W back to device memory (from LDS), then the status.
+    auto finish = [&](int st) {
+        if (in_lds && st != 2) {
+            __syncthreads();
+            for (uint32_t x = tid; x < RR; x += 256) Wg[x] = W[x];
+        }
+        if (tid == 0) *status = st;
+    };
     auto build = [&]() {
         for (uint32_t x = tid; x < RR; x += 256) {
             double v = 1.0;
@@ -208,7 +230,7 @@ __global__ void __launch_bounds__(256) solve_normals_kernel(const double *__rest
                 W[(size_t)r * R + c] = s / V[(size_t)r * R + r];
             }
         }
-        if (tid == 0) *status = 0;
+        finish(0);
         return;
     }
     // Not positive definite: rebuild V and factor P V = L U.
@@ -252,7 +274,7 @@ __global__ void __launch_bounds__(256) solve_normals_kernel(const double *__rest
         __syncthreads();
     }
     if (s_state == 2) {
-        if (tid == 0) *status = 2;
+        finish(2);
         return;
     }
     // W = V^-1: column c solves V x = e_c through the row swaps, L and U.
@@ -277,7 +299,7 @@ __global__ void __launch_bounds__(256) solve_normals_kernel(const double *__rest
             W[(size_t)r * R + c] = s / V[(size_t)r * R + r];
         }
     }
-    if (tid == 0) *status = 1;
+    finish(1);
 }
 
 // A = M W for an nrows x R block of rows (64 x 16 NB output tiles, 16-deep
@@ -353,6 +375,9 @@ __global__ void __launch_bounds__(256) colstat_partial_kernel(const float *__res
         const uint32_t c = g + lane;
         double acc = 0.0;
         if (c < R) {
+            // Unrolled so that eight loads are in flight; the accumulation
+            // order stays the row order.
+#pragma unroll 8
             for (uint64_t i = i0 + phase; i < i1; i += P) {
                 const double a = A[i * stride + c];
                 if (kind == COL_2NORM)
@@ -411,6 +436,16 @@ __global__ void __launch_bounds__(256) colstat_reduce_kernel(const double *__res
 __global__ void __launch_bounds__(256) scale_kernel(float *__restrict__ A, uint64_t nrows, uint32_t R,
                                                     uint32_t stride, const double *__restrict__ lambda) {
     const uint64_t n = nrows * R;
+    if (n < (1ull << 31)) {
+        // 32-bit index arithmetic (x plus the grid stride stays below 2^32):
+        // a 64-bit division per element costs far more than the element's
+        // load and store.
+        for (uint32_t x = blockIdx.x * 256 + threadIdx.x; x < (uint32_t)n; x += gridDim.x * 256) {
+            const uint32_t i = x / R, c = x % R;
+            A[(uint64_t)i * stride + c] = (float)((double)A[(uint64_t)i * stride + c] / lambda[c]);
+        }
+        return;
+    }
     for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (uint64_t)gridDim.x * 256) {
         const uint64_t i = x / R;
         const uint32_t c = (uint32_t)(x % R);
@@ -486,8 +521,9 @@ __global__ void __launch_bounds__(256) fit_kernel(const double *__restrict__ gra
 __global__ void __launch_bounds__(256) to_bf16_kernel(const float *__restrict__ A, uint16_t *__restrict__ B,
                                                       uint64_t nrows, uint32_t R, uint32_t stride) {
     const uint64_t n = nrows * R;
+    const bool idx32 = n < (1ull << 31);   // 32-bit division, as in scale_kernel
     for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (uint64_t)gridDim.x * 256) {
-        const uint64_t off = (x / R) * stride + x % R;
+        const uint64_t off = idx32 ? (uint64_t)((uint32_t)x / R) * stride + (uint32_t)x % R : (x / R) * stride + x % R;
         const uint32_t u = __float_as_uint(A[off]);
         uint16_t h;
         if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu))
@@ -532,7 +568,8 @@ int dense_gram(const float *A, uint64_t nrows, uint32_t R, uint32_t stride, doub
     if (nsplit > cap) nsplit = cap;
     if (nsplit == 0) nsplit = 1;
     uint64_t per = (nrows + nsplit - 1) / nsplit;
-    per = (per + GROWS - 1) / GROWS * GROWS;
+    const uint64_t stage = R <= 16 ? G16ROWS : GROWS;
+    per = (per + stage - 1) / stage * stage;
     if (R <= 16)
         hipLaunchKernelGGL(gram16_partial_kernel, dim3(nsplit), dim3(256), 0, s, A, nrows, R, stride, per,
                            w.partials.as<double>());
@@ -551,7 +588,8 @@ int dense_solve_normals(const double *grams, uint32_t nmodes, uint32_t mode, uin
     PTI_CHECK(w.ensure(R, M), M);
     if (stride < R) PTI_FAIL(PTIERR_SHAPE_MISMATCH, M, "stride < rank");
     if (nrows && Mrhs == A) PTI_FAIL(PTIERR_VALUE_ERROR, M, "right-hand side and result must not alias");
-    hipLaunchKernelGGL(solve_normals_kernel, dim3(1), dim3(256), 0, s, grams, nmodes, mode, R, w.V.as<double>(),
+    const size_t solve_lds = R <= SOLVE_LDS_RANK ? (size_t)2 * R * R * sizeof(double) : 0;
+    hipLaunchKernelGGL(solve_normals_kernel, dim3(1), dim3(256), solve_lds, s, grams, nmodes, mode, R, w.V.as<double>(),
                        w.W.as<double>(), w.piv.as<int>(), w.status.as<int>());
     PTI_HIP(hipGetLastError(), M);
     if (nrows == 0) return 0;

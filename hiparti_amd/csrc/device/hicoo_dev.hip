@@ -372,6 +372,8 @@ __global__ void __launch_bounds__(256) window_hist_kernel(HiCOOArgs h, uint32_t 
 // block-rows are in ascending block coordinate, so the chunk boundaries are
 // unchanged.  Writes the entries of chunk c for NG lane groups in the layout
 // of StreamShape (one wave per chunk).
+// A chunk of at most INTERLEAVE_MAX_RUNS row runs is interleaved: slice g
+// takes its sorted entries g, g + ng, g + 2 ng, ...
 __global__ void __launch_bounds__(256) stream_fill_kernel(HiCOOArgs h, uint32_t nchunks, uint32_t N, uint32_t mode,
                                                           uint32_t ng, const uint32_t *__restrict__ spos,
                                                           const uint32_t *__restrict__ blk_of,
@@ -383,13 +385,22 @@ __global__ void __launch_bounds__(256) stream_fill_kernel(HiCOOArgs h, uint32_t 
     const uint32_t z0 = h.chunk_z[c], cnt = h.chunk_z[c + 1] - z0;
     const uint32_t per = (cnt + ng - 1) / ng;
     const uint32_t G = 64 / ng;
+    auto row_of = [&](uint32_t z) { return (h.einds[(uint64_t)z * EW + mode / 4] >> ((mode % 4) * 8)) & 0xFFu; };
+    uint32_t breaks = 0;   // row changes between neighbouring sorted entries (the same in every lane)
+    for (uint32_t kb = 1; kb < cnt; kb += 64) {
+        const uint32_t k = kb + lane;
+        const bool brk = k < cnt && row_of(spos[z0 + k]) != row_of(spos[z0 + k - 1]);
+        breaks += (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(brk));
+    }
+    const bool interleave = breaks < INTERLEAVE_MAX_RUNS;
     for (uint32_t e = lane; e < CHUNK; e += 64) {
         const uint32_t g = (e % 64) / G, j = (e / 64) * G + e % G;   // e == stream_entry(G, g, j)
         uint32_t w[12];
         for (uint32_t k = 0; k < 12; ++k) w[k] = 0u;
         w[N] = DUMMY_ROW;
-        if (j < per && g * per + j < cnt) {
-            const uint32_t z = spos[z0 + g * per + j];
+        const uint32_t k = interleave ? j * ng + g : g * per + j;
+        if (j < per && k < cnt) {
+            const uint32_t z = spos[z0 + k];
             const uint64_t b = blk_of[z];
             auto elem = [&](uint32_t m) { return (h.einds[(uint64_t)z * EW + m / 4] >> ((m % 4) * 8)) & 0xFFu; };
             for (uint32_t i = 0; i + 1 < N; ++i) {
