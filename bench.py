@@ -107,20 +107,28 @@ KERNEL = "hicoo_stream_kernel<3, 4, 8, float, 8, 0>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hicoo_stream_kernel_zipf100M_R32_counters.md")
 
 
-KERNEL_SOURCE = os.path.join(ROOT, "hiparti_amd", "csrc", "device", "mttkrp_kernels.hpp")
+# The kernel's source and the code that lays out the stream forms it reads:
+# a change to either can move the traffic.
+KERNEL_SOURCES = [os.path.join(ROOT, "hiparti_amd", "csrc", "device", f)
+                  for f in ("mttkrp_kernels.hpp", "launch.hpp", "hicoo_dev.hip", "coo_dev.hip")]
 
 
 def kernel_source_sha():
+    """SHA-256 over the concatenated KERNEL_SOURCES."""
     import hashlib
-    with open(KERNEL_SOURCE, "rb") as f:
-        return hashlib.sha256(f.read()).hexdigest()
+    h = hashlib.sha256()
+    for path in KERNEL_SOURCES:
+        with open(path, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
 
 
 def measured_traffic(which="traffic"):
     """HBM bytes per launch of the stream kernel on this workload, from the
     committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE,
     calibrated on a plain copy), when it was taken on the current kernel:
-    same kernel name and the same SHA-256 of the kernel source.  which:
+    same kernel name and the same SHA-256 of the kernel and stream-form
+    sources (KERNEL_SOURCES).  which:
     "traffic" (FP32 HiCOO), "bf16_traffic" or "coo_traffic"."""
     try:
         text = open(TRAFFIC_FILE).read()

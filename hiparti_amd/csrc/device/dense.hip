@@ -162,8 +162,8 @@ __global__ void __launch_bounds__(256) gram_reduce_kernel(const double *__restri
 // status: 1 int.  For R <= SOLVE_LDS_RANK the factorization and the inverse
 // run in dynamic LDS (2 R^2 doubles, which the launch provides) and only W is
 // written back: every elimination step is a workgroup barrier, and with the
-// matrices in device memory each step also waits on L2 round trips (37 us
-// per solve at R = 16).
+// matrices in device memory each step also waited on L2 round trips (37 us
+// per solve at R = 16 there, 25 us in LDS).
 constexpr uint32_t SOLVE_LDS_RANK = 64;
 __global__ void __launch_bounds__(256) solve_normals_kernel(const double *__restrict__ grams, uint32_t nmodes,
                                                             uint32_t mode, uint32_t R, double *Vg, double *Wg,
