@@ -103,7 +103,7 @@ def copy_calibration(gib):
     print(json.dumps({"copy_calibration_bytes_per_copy": n * 4, "copies": 3}), flush=True)
 
 
-KERNEL = "hicoo_stream_kernel<3, 4, 8, float, 8, 0>"
+KERNEL = "hicoo_stream_kernel<3, 4, 8, float, 8, 0, false>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hicoo_stream_kernel_zipf100M_R32_counters.md")
 
 
@@ -346,7 +346,7 @@ def coo_leg(X, ndims, dfacs, facs_np, outs, R, args, nthreads, rows=None):
     res = {"ms_per_step": round(ms, 4), "per_mode_ms": [round(x, 4) for x in per_mode],
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("coo_traffic"),
-                        "kernel": "hicoo_stream_kernel<3, 4, 8, float, 8, 0> on the COO copy's 128-row blocks "
+                        "kernel": "hicoo_stream_kernel<3, 4, 8, float, 8, 0, false> on the COO copy's 128-row blocks "
                                   "(+ output zero fill)",
                         "alg_bytes_per_launch": int(alg)}}
     if not args.no_cpu:

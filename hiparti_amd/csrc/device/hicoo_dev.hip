@@ -60,6 +60,7 @@ struct ptiDevHiCOOMode {
         pti::dev::Buffer ent;
         uint32_t ng = 0;
         uint64_t last_use = 0;
+        bool packed = false;   // one plane of packed entries (StreamShape<N, true>)
     };
     mutable std::mutex stream_mu;
     mutable StreamForm forms[2];
@@ -373,15 +374,17 @@ __global__ void __launch_bounds__(256) window_hist_kernel(HiCOOArgs h, uint32_t 
 // unchanged.  Writes the entries of chunk c for NG lane groups in the layout
 // of StreamShape (one wave per chunk).
 // A chunk of at most INTERLEAVE_MAX_RUNS row runs is interleaved: slice g
-// takes its sorted entries g, g + ng, g + 2 ng, ...
+// takes its sorted entries g, g + ng, g + 2 ng, ...  packed (orders 4 and 5):
+// one plane of four words, StreamShape<N, true>.
 __global__ void __launch_bounds__(256) stream_fill_kernel(HiCOOArgs h, uint32_t nchunks, uint32_t N, uint32_t mode,
-                                                          uint32_t ng, const uint32_t *__restrict__ spos,
+                                                          uint32_t ng, uint32_t packed,
+                                                          const uint32_t *__restrict__ spos,
                                                           const uint32_t *__restrict__ blk_of,
                                                           uint4 *__restrict__ ent) {
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t c = blockIdx.x * 4 + wid;
     if (c >= nchunks) return;   // whole wave
-    const uint32_t EW = (N + 3) / 4, NP = (N + 4) / 4;
+    const uint32_t EW = (N + 3) / 4, NP = packed ? 1u : (N + 4) / 4;
     const uint32_t z0 = h.chunk_z[c], cnt = h.chunk_z[c + 1] - z0;
     const uint32_t per = (cnt + ng - 1) / ng;
     const uint32_t G = 64 / ng;
@@ -409,6 +412,14 @@ __global__ void __launch_bounds__(256) stream_fill_kernel(HiCOOArgs h, uint32_t 
             }
             w[N - 1] = __float_as_uint(h.vals[z]);
             w[N] = elem(mode);
+        }
+        if (packed) {
+            // {c0 | row << 24, c1, c2, value}; order 5 adds c3's low byte to
+            // word 1 and its high byte to word 2.
+            const uint32_t p1 = N == 5 ? w[1] | (w[3] << 24) : w[1];
+            const uint32_t p2 = N == 5 ? w[2] | ((w[3] >> 8) << 24) : w[2];
+            ent[(uint64_t)c * CHUNK + e] = make_uint4(w[0] | (w[N] << 24), p1, p2, w[N - 1]);
+            continue;
         }
         for (uint32_t q = 0; q < NP; ++q)
             ent[((uint64_t)c * NP + q) * CHUNK + e] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
@@ -651,13 +662,14 @@ namespace {
 // The stream form of the copy of `mode` for ng lane groups (*ent): a cached
 // one, or one built now in the free or least recently used slot.
 int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t s, const char *M,
-                  const uint4 **ent) {
+                  const uint4 **ent, bool *packed) {
     const ptiDevHiCOOMode &c = t->mc[mode];
     std::lock_guard<std::mutex> lk(c.stream_mu);
     for (auto &f : c.forms)
         if (f.ng == ng) {
             f.last_use = ++c.use_clock;
             *ent = f.ent.as<uint4>();
+            *packed = f.packed;
             return 0;
         }
     ptiDevHiCOOMode::StreamForm &dst =
@@ -667,7 +679,14 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
     if (dst.ent.ptr) PTI_HIP(hipDeviceSynchronize(), M);
     dst.ng = 0;
     const uint32_t N = t->nmodes;
-    const uint32_t NP = (N + 4) / 4;
+    // Packed entries (StreamShape<N, true>): orders 4 and 5, rows of at most
+    // 7 bits, gathered coordinates below 2^24, and for order 5 a last
+    // gathered mode of at most 2^16 rows.
+    bool pack = (N == 4 || N == 5) && t->sb_bits <= 7;
+    for (uint32_t m = 0; m < N; ++m)
+        if (m != mode && t->ndims[m] > (1u << 24)) pack = false;
+    if (N == 5 && t->ndims[mode == 4 ? 3 : 4] > (1u << 16)) pack = false;
+    const uint32_t NP = pack ? 1u : (N + 4) / 4;
     PTI_HIP(dst.ent.alloc((size_t)c.nchunks * NP * CHUNK * sizeof(uint4)), M);
     // Inside a row, nonzeros follow the coordinate of the largest non-target
     // mode, whose factor rows are the likeliest cache misses.
@@ -710,7 +729,7 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
                                       spos.as<uint32_t>(), (size_t)nnz, 0, kbits, s),
             M);
     hipLaunchKernelGGL(stream_fill_kernel, grid, dim3(256), 0, s, hicoo_args(t, mode), c.nchunks, N, mode, ng,
-                       spos.as<uint32_t>(), blk_of.as<uint32_t>(), dst.ent.as<uint4>());
+                       pack ? 1u : 0u, spos.as<uint32_t>(), blk_of.as<uint32_t>(), dst.ent.as<uint4>());
     PTI_HIP(hipGetLastError(), M);
     if (!c.unit_order.ptr) {
         if (windowed)
@@ -722,7 +741,9 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
     PTI_HIP(hipStreamSynchronize(s), M);
     dst.ng = ng;
     dst.last_use = ++c.use_clock;
+    dst.packed = pack;
     *ent = dst.ent.as<uint4>();
+    *packed = pack;
     return 0;
 }
 }  // namespace
@@ -840,7 +861,8 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
         for (uint32_t i = 0; i < N; ++i) a.order[i] = mats_order[i];
         if (use_stream) {
             const uint4 *ent = nullptr;
-            PTI_CHECK(ensure_stream(t, mode, 64u / sh.G, s, M, &ent), M);
+            bool packed = false;
+            PTI_CHECK(ensure_stream(t, mode, 64u / sh.G, s, M, &ent, &packed), M);
             const ptiDevHiCOOMode &c = t->mc[mode];
             StreamArgs h{};
             h.ent = ent;
@@ -853,6 +875,13 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
             h.coord24 = coords_fit_u24(t->ndims, N, mode, stride * esz);
             h.unit_order = c.unit_order.as<uint32_t>();
             resident_factors(t, mode, sh, esz, c.nunits, h.res_rows);
+            if (packed && !h.coord24) {
+                // Packed entries need the 24-bit row offsets (row pitch below
+                // 16 MB), which ignore the fields in the coordinates' high byte.
+                PTI_HIP(launch_hicoo(N, hicoo_args(t, mode), a, sh, bf16, s), M);
+                continue;
+            }
+            h.packed = packed ? 1u : 0u;
             PTI_HIP(launch_stream(N, h, a, sh, bf16, s), M);
         } else {
             PTI_HIP(launch_hicoo(N, hicoo_args(t, mode), a, sh, bf16, s), M);

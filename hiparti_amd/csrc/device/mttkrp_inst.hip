@@ -59,17 +59,31 @@ hipError_t hicoo_vg(const HiCOOArgs &h, const FactorArgs &a, bool bf16, hipStrea
 
 // The stream kernel with NRES LDS-resident factors (orders 4 and 5, four
 // ranks per lane); dynamic LDS beyond 64 KB is allowed explicitly.
-template <int V, int G, typename FT, int NRES>
+template <int V, int G, typename FT, int NRES, bool PACK>
 hipError_t stream_res(const StreamArgs &h, const FactorArgs &a, size_t lds, hipStream_t s) {
-    const void *fn = reinterpret_cast<const void *>(&hicoo_stream_kernel<N, V, G, FT, 8, NRES>);
+    const void *fn = reinterpret_cast<const void *>(&hicoo_stream_kernel<N, V, G, FT, 8, NRES, PACK>);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((hicoo_stream_kernel<N, V, G, FT, 8, NRES>), dim3(h.nunits), dim3(512), lds, s, h, a);
+    hipLaunchKernelGGL((hicoo_stream_kernel<N, V, G, FT, 8, NRES, PACK>), dim3(h.nunits), dim3(512), lds, s, h, a);
     return hipGetLastError();
 }
 
+template <int V, int G, bool PACK>
+hipError_t stream_vgp(const StreamArgs &h, const FactorArgs &a, bool bf16, hipStream_t s);
+
+// Packed forms exist for orders 4 and 5 only (StreamShape).
 template <int V, int G>
 hipError_t stream_vg(const StreamArgs &h, const FactorArgs &a, bool bf16, hipStream_t s) {
+    if constexpr (N == 4 || N == 5) {
+        if (h.packed) return stream_vgp<V, G, true>(h, a, bf16, s);
+    } else {
+        if (h.packed) return hipErrorInvalidValue;
+    }
+    return stream_vgp<V, G, false>(h, a, bf16, s);
+}
+
+template <int V, int G, bool PACK>
+hipError_t stream_vgp(const StreamArgs &h, const FactorArgs &a, bool bf16, hipStream_t s) {
     if constexpr (G * V > 64 || !stream_supported(N, G)) {
         return hipErrorInvalidValue;  // the LDS tile holds at most 64 ranks
     } else {
@@ -84,11 +98,11 @@ hipError_t stream_vg(const StreamArgs &h, const FactorArgs &a, bool bf16, hipStr
                 // Order 4 takes one resident factor, order 5 up to two.
                 if (N == 4 && nres > 1) return hipErrorInvalidValue;
                 if (bf16) {
-                    if (nres == 1) return stream_res<V, G, uint16_t, 1>(h, a, lds, s);
-                    if constexpr (N == 5) return stream_res<V, G, uint16_t, 2>(h, a, lds, s);
+                    if (nres == 1) return stream_res<V, G, uint16_t, 1, PACK>(h, a, lds, s);
+                    if constexpr (N == 5) return stream_res<V, G, uint16_t, 2, PACK>(h, a, lds, s);
                 }
-                if (nres == 1) return stream_res<V, G, float, 1>(h, a, lds, s);
-                if constexpr (N == 5) return stream_res<V, G, float, 2>(h, a, lds, s);
+                if (nres == 1) return stream_res<V, G, float, 1, PACK>(h, a, lds, s);
+                if constexpr (N == 5) return stream_res<V, G, float, 2, PACK>(h, a, lds, s);
             }
             return hipErrorInvalidValue;
         }
@@ -97,12 +111,12 @@ hipError_t stream_vg(const StreamArgs &h, const FactorArgs &a, bool bf16, hipStr
         if constexpr (V == 8) {
             // Eight ranks per lane exist only for BF16 factors (16-byte pieces).
             if (!bf16) return hipErrorInvalidValue;
-            hipLaunchKernelGGL((hicoo_stream_kernel<N, V, G, uint16_t, 8>), grid, dim3(512), lds, s, h, a);
+            hipLaunchKernelGGL((hicoo_stream_kernel<N, V, G, uint16_t, 8, 0, PACK>), grid, dim3(512), lds, s, h, a);
         } else {
             if (bf16)
-                hipLaunchKernelGGL((hicoo_stream_kernel<N, V, G, uint16_t, 8>), grid, dim3(512), lds, s, h, a);
+                hipLaunchKernelGGL((hicoo_stream_kernel<N, V, G, uint16_t, 8, 0, PACK>), grid, dim3(512), lds, s, h, a);
             else
-                hipLaunchKernelGGL((hicoo_stream_kernel<N, V, G, float, 8>), grid, dim3(512), lds, s, h, a);
+                hipLaunchKernelGGL((hicoo_stream_kernel<N, V, G, float, 8, 0, PACK>), grid, dim3(512), lds, s, h, a);
         }
         return hipGetLastError();
     }

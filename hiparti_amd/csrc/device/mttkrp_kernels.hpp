@@ -554,8 +554,8 @@ constexpr size_t hicoo_stage_bytes(int W) {
 // those of the copy.  An entry has N + 1 words: the coordinates of the N - 1
 // non-target modes in ascending mode order, the value bits and the output row
 // inside the block-row; positions past a slice's end are all-zero with row
-// DUMMY_ROW.  Entries are stored in NP = ceil((N + 1) / 4) planes of 16-byte
-// words.  Chunk c owns NP * CHUNK 16-byte words starting at c * NP * CHUNK,
+// DUMMY_ROW (packed forms of orders 4 and 5: four words, StreamShape).
+// Entries are stored in NP = ceil(words / 4) planes of 16-byte words.  Chunk c owns NP * CHUNK 16-byte words starting at c * NP * CHUNK,
 // plane q first at offset q * CHUNK: its nonzeros are cut into NG = 64 / G
 // slices of per = ceil(cnt / NG) positions (G lanes per slice) and position j
 // of slice g is entry stream_entry(G, g, j) = 64 * (j / G) + G * g + j % G, so
@@ -565,9 +565,16 @@ __host__ __device__ constexpr uint32_t stream_entry(uint32_t G, uint32_t g, uint
     return 64u * (j / G) + G * g + j % G;
 }
 
-template <int N>
+// Packed forms (orders 4 and 5, every gathered coordinate below 2^24, rows
+// of at most 7 bits): four words, one plane, half the bytes of the two-plane
+// entry.  Order 4: {c0 | row << 24, c1, c2, value}; order 5: {c0 | row << 24,
+// c1 | (c3 & 255) << 24, c2 | (c3 >> 8) << 24, value}, which needs the last
+// gathered mode to have at most 2^16 rows.  The walk's 24-bit multiplies
+// ignore the high bytes, so only the row and c3 take shifts to unpack.
+template <int N, bool PACK = false>
 struct StreamShape {
-    static constexpr int WORDS = N + 1;
+    static_assert(!PACK || N == 4 || N == 5, "packed entries exist for orders 4 and 5");
+    static constexpr int WORDS = PACK ? 4 : N + 1;
     static constexpr int NP = (WORDS + 3) / 4;
 };
 
@@ -590,6 +597,8 @@ struct StreamArgs {
     // LDS-resident factors (kernels instantiated with NRES > 0): the rows of
     // the last NRES gathered modes, which the workgroup copies whole into LDS.
     uint32_t res_rows[2];
+    // 1 when the stream form holds packed entries (StreamShape<N, true>).
+    uint32_t packed;
 };
 
 // Byte offset of the LDS-resident factors behind a tile of `rows` x tp floats.
@@ -702,17 +711,20 @@ struct TileSink {
 // from their whole copies in LDS at res (G * V elements per row, rows of the
 // first resident factor, then of the second), every lane of every position,
 // so the walk stays free of branches.
-template <int N, int V, int G, int D, typename FT, bool U24, bool RESID = false, int NRES = 0, typename Sink>
+// PACK: the entries are packed (StreamShape<N, true>); needs U24.
+template <int N, int V, int G, int D, typename FT, bool U24, bool RESID = false, int NRES = 0, bool PACK = false,
+          typename Sink>
 __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__restrict__ ent, const Sink &sink,
                                             int lane, const float *trow = nullptr, uint32_t trow_last = 0,
                                             double *esq_sum = nullptr, const char *res = nullptr,
                                             uint32_t res1_off = 0) {
+    static_assert(!PACK || U24, "packed coordinate words carry other fields in their high byte");
     // Sum of squared residuals (RESID with esq_sum): kept in a register and
     // added to *esq_sum once at the end.
     double esq = 0.0;
     using VT = typename Vec<FT, V>::T;
     constexpr int K = N - 1;
-    constexpr int NP = StreamShape<N>::NP;
+    constexpr int NP = StreamShape<N, PACK>::NP;
     constexpr int NG = 64 / G;
     constexpr int PMAX = CHUNK / NG;
     constexpr int NB = CHUNK / 64;
@@ -738,6 +750,13 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
         constexpr int p = decltype(P)::value;
         return group_word<G, p % G>(blk[w / 4][p / G][w % 4], gbase);
     };
+    // Coordinate word of gathered position i (packed order 5: c3 sits in the
+    // high bytes of words 1 and 2; the other high bytes are ignored by the
+    // 24-bit multiplies below).
+    auto coord = [&](auto P, int i) -> uint32_t {
+        if (PACK && K == 4 && i == 3) return (word(P, 1) >> 24) | ((word(P, 2) >> 24) << 8);
+        return word(P, i);
+    };
     VT raw[D][K];
     float vv[D];
     uint32_t rr[D];
@@ -747,14 +766,19 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
 #pragma unroll
         for (int i = 0; i < K; ++i) {
             if (i >= K - NRES) {
-                const uint32_t off = (i == K - NRES ? 0u : res1_off) + word(P, i) * RRB + colb;
+                const uint32_t off = (i == K - NRES ? 0u : res1_off) + __umul24(coord(P, i), RRB) + colb;
                 raw[p % D][i] = *static_cast<const VT *>(__builtin_assume_aligned(res + off, sizeof(VT)));
             } else {
-                raw[p % D][i] = *reinterpret_cast<const VT *>(f[i] + (uint64_t)row_offset<U24>(word(P, i), rowb, colb));
+                raw[p % D][i] = *reinterpret_cast<const VT *>(f[i] + (uint64_t)row_offset<U24>(coord(P, i), rowb, colb));
             }
         }
-        vv[p % D] = __uint_as_float(word(P, K));
-        rr[p % D] = word(P, K + 1);
+        if constexpr (PACK) {
+            vv[p % D] = __uint_as_float(word(P, 3));
+            rr[p % D] = word(P, 0) >> 24;
+        } else {
+            vv[p % D] = __uint_as_float(word(P, K));
+            rr[p % D] = word(P, K + 1);
+        }
     };
     static_for<0, (D < PMAX ? D : PMAX)>(issue);
     float acc[V];
@@ -857,11 +881,13 @@ struct StreamConfig {
 // NRES > 0: the workgroup first copies the last NRES gathered factors whole
 // (h.res_rows rows each, the rank tile's G * V columns, zero past the rank)
 // into LDS behind the tile; the walk then reads them there.
-template <int N, int V, int G, typename FT, int W, int NRES = 0>
+// PACK: the form holds packed entries (orders 4 and 5; the host sends such
+// forms here only with coord24).
+template <int N, int V, int G, typename FT, int W, int NRES = 0, bool PACK = false>
 __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, FT>::OCC))
     hicoo_stream_kernel(StreamArgs h, FactorArgs a) {
     constexpr int D = StreamConfig<N, V, FT>::D;
-    constexpr int NP = StreamShape<N>::NP;
+    constexpr int NP = StreamShape<N, PACK>::NP;
     extern __shared__ float tile[];  // (1 << sb_bits) rows x TP floats [, resident factors]
     constexpr int GV = G * V;
     constexpr int TP = GV + 1;
@@ -896,11 +922,12 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, FT>::OCC))
     __syncthreads();
     const uint32_t c0 = h.unit_chunk[u], c1 = h.unit_chunk[u + 1];
     const uint32_t w0 = __builtin_amdgcn_readfirstlane(wid);
-    if (h.coord24) {
+    if (PACK || h.coord24) {
         for (uint32_t c = c0 + w0; c < c1; c += W)
-            stream_walk<N, V, G, D, FT, true, false, NRES>(a, h.ent + (uint64_t)c * NP * CHUNK, TileSink{tile, TP},
-                                                            lane, nullptr, 0, nullptr, res, res1_off);
-    } else {
+            stream_walk<N, V, G, D, FT, true, false, NRES, PACK>(a, h.ent + (uint64_t)c * NP * CHUNK,
+                                                                  TileSink{tile, TP}, lane, nullptr, 0, nullptr, res,
+                                                                  res1_off);
+    } else if constexpr (!PACK) {
         for (uint32_t c = c0 + w0; c < c1; c += W)
             stream_walk<N, V, G, D, FT, false, false, NRES>(a, h.ent + (uint64_t)c * NP * CHUNK, TileSink{tile, TP},
                                                              lane, nullptr, 0, nullptr, res, res1_off);

@@ -379,6 +379,52 @@ def test_stream_kernel_with_coordinates_beyond_24_bits():
             assert rel_fro(got, want) <= TOL, (name, mode)
 
 
+@pytest.mark.parametrize("ndims", [[(1 << 24) + 40, 300, 200, 50], [(1 << 24) + 40, 300, 200, 50, 20],
+                                   [3000, 300, 200, 50, 70000], [3000, 300, 200, 50, 1000]])
+def test_packed_and_two_plane_stream_forms(ndims):
+    """Order-4 and order-5 stream forms pack each entry into one 16-byte word
+    (the output row in the first coordinate's high byte; for order 5 the last
+    gathered coordinate split over the second and third words' high bytes)
+    only where every gathered coordinate is below 2^24 and, for order 5, the
+    last gathered mode has at most 2^16 rows; other forms keep two planes.
+    Here the modes that gather the (2^24 + 40)-row mode 0 (some coordinates
+    at and above 2^24) and, in the third tensor, modes 0-3 (whose last
+    gathered mode has 70000 rows) take two planes; the other modes pack, in
+    the fourth tensor with last gathered coordinates up to 999 (both bytes
+    used), some with the short trailing factor resident in LDS.  HiCOO FP32
+    and BF16 against the FP64 definition."""
+    import torch
+    from hiparti_amd.device import DevHiCOO
+    N = len(ndims)
+    R = 16
+    rng = np.random.default_rng(31)
+    nnz = 40000
+    inds = np.stack([rng.integers(0, d, nnz) for d in ndims]).astype(np.uint32)
+    if ndims[0] > (1 << 24):
+        inds[0, : nnz // 3] = rng.integers((1 << 24) - 8, ndims[0], nnz // 3)
+    key = np.zeros(nnz, dtype=np.uint64)
+    for m in range(N):
+        key = key * np.uint64(ndims[m] if m else 1) + inds[m].astype(np.uint64)
+    inds = inds[:, np.unique(key, return_index=True)[1]]
+    t = COOTensor.from_arrays(ndims, inds, rng.random(inds.shape[1]).astype(np.float32))
+    dh = DevHiCOO(t.toHiCOO(7, 10, 10))
+    inds, vals = t.inds().astype(np.int64), t.values().astype(np.float64)
+    gen = torch.Generator(device="cuda").manual_seed(8)
+    dfacs = [torch.rand((d, R), generator=gen, device="cuda") for d in ndims]
+    bf = [f.to(torch.bfloat16) for f in dfacs]
+    for facs, name in ((dfacs, "fp32"), (bf, "bf16")):
+        rows = [facs[m][torch.from_numpy(inds[m]).cuda()].double().cpu().numpy() for m in range(N)]
+        for mode in range(N):
+            want = np.zeros((ndims[mode], R))
+            prod = vals[:, None].copy()
+            for m in range(N):
+                if m != mode:
+                    prod = prod * rows[m]
+            np.add.at(want, inds[mode], prod)
+            got = (dh.mttkrp(facs, mode) if name == "fp32" else dh.mttkrp_bf16(facs, mode)).cpu().numpy()[:, :R]
+            assert rel_fro(got, want) <= TOL, (N, name, mode)
+
+
 def test_stream_paths_on_extreme_row_distributions(oracle):
     """The stream kernels' run sums, chunk padding and unit write-out on
     extreme shapes, HiCOO and COO against the FP64 oracle: every nonzero in
