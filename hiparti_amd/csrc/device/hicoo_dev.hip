@@ -26,7 +26,8 @@
 //
 // Each mode copy also gets a stream form on its first MTTKRP (ensure_stream):
 // the nonzeros of each block-row sorted by output row and decoded into
-// (N + 1)-word entries laid out for the lane groups of the rank shape
+// (N + 1)-word entries (four-word packed entries for orders 4 and 5 where
+// they fit) laid out for the lane groups of the rank shape
 // (StreamShape in mttkrp_kernels.hpp), 4 KB per chunk and 16-byte plane, read
 // by hicoo_stream_kernel.
 #include <algorithm>
