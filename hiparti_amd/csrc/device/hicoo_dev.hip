@@ -67,6 +67,9 @@ struct ptiDevHiCOOMode {
     mutable StreamForm forms[2];
     mutable uint64_t use_clock = 0;
     mutable pti::dev::Buffer unit_order;   // units by decreasing chunk count
+    // One byte per block-row: 1 when a single unit owns it (it then writes
+    // the block-row's output rows whole), built on first use.
+    mutable pti::dev::Buffer br_owned;
 };
 
 struct ptiDevHiCOO {
@@ -749,6 +752,54 @@ int ensure_stream(const ptiDevHiCOO *t, uint32_t mode, uint32_t ng, hipStream_t 
 }
 }  // namespace
 
+// Block-rows owned by a single unit (flags zeroed by the caller).
+__global__ void __launch_bounds__(256) owned_rows_kernel(const uint32_t *__restrict__ unit_br,
+                                                         const uint32_t *__restrict__ unit_excl, uint32_t nunits,
+                                                         uint8_t *__restrict__ owned) {
+    const uint32_t u = blockIdx.x * 256 + threadIdx.x;
+    if (u < nunits && unit_excl[u]) owned[unit_br[u]] = 1;
+}
+
+// Zeroes the output rows whose block-row no single unit owns: one wave per
+// row, rows of `width` floats.
+__global__ void __launch_bounds__(256) zero_shared_rows_kernel(float *__restrict__ out, uint32_t nrows,
+                                                               uint32_t width, const uint8_t *__restrict__ owned,
+                                                               uint32_t sb) {
+    const uint32_t wave = (blockIdx.x * 256 + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    const uint32_t nwaves = (gridDim.x * 256) >> 6;
+    for (uint32_t r = wave; r < nrows; r += nwaves) {
+        if (owned[r >> sb]) continue;
+        float *row = out + (uint64_t)r * width;
+        for (uint32_t c = lane; c < width; c += 64) row[c] = 0.f;
+    }
+}
+
+int ensure_owned_rows(const ptiDevHiCOO *t, uint32_t mode, hipStream_t s, const char *M) {
+    const ptiDevHiCOOMode &c = t->mc[mode];
+    std::lock_guard<std::mutex> lk(c.stream_mu);
+    if (c.br_owned.ptr) return 0;
+    const uint64_t nbr = ((uint64_t)t->ndims[mode] + (1u << t->sb_bits) - 1) >> t->sb_bits;
+    Buffer owned;
+    PTI_HIP(owned.alloc((nbr + 3) / 4 * 4), M);
+    PTI_HIP(zero_floats(owned.as<float>(), (nbr + 3) / 4, s), M);   // all-zero bits
+    if (c.nunits) {
+        hipLaunchKernelGGL(owned_rows_kernel, dim3((c.nunits + 255) / 256), dim3(256), 0, s, c.unit_br.as<uint32_t>(),
+                           c.unit_excl.as<uint32_t>(), c.nunits, owned.as<uint8_t>());
+        PTI_HIP(hipGetLastError(), M);
+    }
+    c.br_owned = std::move(owned);
+    return 0;
+}
+
+hipError_t zero_shared_rows(float *out, uint32_t nrows, uint32_t width, const uint8_t *owned, uint32_t sb,
+                            hipStream_t s) {
+    uint64_t blocks = ((uint64_t)nrows + 3) / 4;   // four rows (waves) per workgroup
+    if (blocks > 16384) blocks = 16384;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_shared_rows_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, out, nrows, width, owned, sb);
+    return hipGetLastError();
+}
+
 // LDS-resident factors for the stream kernel (orders 4 and 5, four ranks per
 // lane): the last gathered modes, in ascending mode order, whose
 // whole rank-tile copies (rows x G * V elements) fit beside the LDS tile in
@@ -801,8 +852,10 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
     if (!order_is_permutation(N, mats_order, mode))
         PTI_FAIL(PTIERR_VALUE_ERROR, M, "mats_order[1..N-1] must be a permutation of the other modes");
     if (t->nnz && !t->mc[mode].ready) PTI_FAIL(PTIERR_VALUE_ERROR, M, "mode was not uploaded");
-    PTI_HIP(zero_floats(out, (uint64_t)t->ndims[mode] * out_stride, s), M);
-    if (t->nnz == 0) return 0;
+    if (t->nnz == 0) {
+        PTI_HIP(zero_floats(out, (uint64_t)t->ndims[mode] * out_stride, s), M);
+        return 0;
+    }
     // 64 ranks per LDS tile; 32 for orders >= 6, whose larger staging area
     // would otherwise push a workgroup past 64 KB of LDS.
     const ptiIndex tile = N >= 6 ? 32 : 64;
@@ -810,12 +863,14 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
     bool fits = true;   // the stream kernel gathers with 32-bit byte offsets
     for (uint32_t i = 1; i < N; ++i)
         if ((uint64_t)t->ndims[mats_order[i]] * stride * esz >= (1ull << 32)) fits = false;
+    const bool coord24 = coords_fit_u24(t->ndims, N, mode, stride * esz);
+    // Lane shape and kernel of the column tile starting at c0 (tiles in
+    // order: `first` is the first tile's shape).
     Shape first{0, 0};
-    for (ptiIndex c0 = 0; c0 < R; c0 += tile) {
+    auto plan_tile = [&](ptiIndex c0, Shape &sh, bool &use_stream) -> int {
         const ptiIndex Rt = std::min<ptiIndex>(tile, R - c0);
         // Column tiles start at multiples of 32 elements, so they keep the
         // alignment the vector width V relies on.
-        Shape sh;
         if (!choose_shape(Rt, stride, out_stride, sh)) PTI_FAIL(PTIERR_VALUE_ERROR, M, "unsupported rank tile");
         // BF16 factors on the stream kernel: eight ranks (16 bytes) per lane
         // when the rank tile needs more than four such lanes (lane groups of
@@ -845,13 +900,41 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
             else
                 keep_form = sh.G == first.G;
         }
-        const bool use_stream = fits && stream_supported(N, sh.G) && keep_form;
+        use_stream = fits && stream_supported(N, sh.G) && keep_form;
         // The tile kernel takes at most four ranks per lane.
         if (!use_stream && sh.V == 8) choose_shape(Rt, stride, out_stride, sh);
         // The kernel stages factor-row offsets as 32-bit counts of V-wide vectors.
         for (uint32_t i = 1; i < N; ++i)
             if ((uint64_t)t->ndims[mats_order[i]] * (stride / sh.V) >= (1ull << 32))
                 PTI_FAIL(PTIERR_VALUE_ERROR, M, "factor matrix exceeds 2^32 row-vector slots");
+        return 0;
+    };
+    // Whole rows from single-unit block-rows: when every tile runs the stream
+    // kernel and the last tile's lanes reach the end of the output row
+    // (padding included), the unit that owns a block-row alone writes all
+    // of its rows and columns, zeros included, so the zero fill only covers
+    // block-rows that several units share or that hold no nonzero.  (A
+    // packed form with a 16 MB row pitch or more would go to the tile kernel,
+    // so orders 4 and 5 then keep the full fill.)
+    bool whole_rows = coord24 || (N != 4 && N != 5);
+    for (ptiIndex c0 = 0; c0 < R && whole_rows; c0 += tile) {
+        Shape sh;
+        bool use_stream = false;
+        PTI_CHECK(plan_tile(c0, sh, use_stream), M);
+        if (!use_stream || (c0 + tile >= R && out_stride > c0 + (ptiIndex)(sh.G * sh.V))) whole_rows = false;
+    }
+    if (whole_rows) {
+        PTI_CHECK(ensure_owned_rows(t, mode, s, M), M);
+        const ptiDevHiCOOMode &c = t->mc[mode];
+        PTI_HIP(zero_shared_rows(out, t->ndims[mode], out_stride, c.br_owned.as<uint8_t>(), t->sb_bits, s), M);
+    } else {
+        PTI_HIP(zero_floats(out, (uint64_t)t->ndims[mode] * out_stride, s), M);
+    }
+    for (ptiIndex c0 = 0; c0 < R; c0 += tile) {
+        const ptiIndex Rt = std::min<ptiIndex>(tile, R - c0);
+        Shape sh;
+        bool use_stream = false;
+        PTI_CHECK(plan_tile(c0, sh, use_stream), M);
         FactorArgs a;
         for (uint32_t m = 0; m < N; ++m) a.fac[m] = static_cast<const char *>(facs[m]) + (size_t)c0 * esz;
         a.out = out + c0;
@@ -859,6 +942,7 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
         a.stride = stride;
         a.out_stride = out_stride;
         a.mode = mode;
+        a.out_cols = whole_rows ? out_stride - c0 : 0u;
         for (uint32_t i = 0; i < N; ++i) a.order[i] = mats_order[i];
         if (use_stream) {
             const uint4 *ent = nullptr;
@@ -873,7 +957,7 @@ int run_hicoo(const ptiDevHiCOO *t, const void *const facs[], float *out, ptiInd
             h.nunits = c.nunits;
             h.sb_bits = t->sb_bits;
             h.nrows = t->ndims[mode];
-            h.coord24 = coords_fit_u24(t->ndims, N, mode, stride * esz);
+            h.coord24 = coord24;
             h.unit_order = c.unit_order.as<uint32_t>();
             resident_factors(t, mode, sh, esz, c.nunits, h.res_rows);
             if (packed && !h.coord24) {

@@ -79,6 +79,10 @@ struct FactorArgs {
     uint32_t out_stride;         // output row stride (elements)
     uint32_t mode;               // target mode
     uint32_t order[MAX_MODES];   // order[1..N-1]: factor modes in product order
+    // > 0 (stream kernel): a unit that owns its block-row alone writes the
+    // rows whole, columns [0, min(G * V, out_cols)) of out, zeros included
+    // (columns past R as zeros), so the caller need not zero those rows.
+    uint32_t out_cols = 0;
 };
 
 // One target mode's copy of a HiCOO tensor (see the header comment).
@@ -524,13 +528,19 @@ __global__ void __launch_bounds__(W * 64, 6) hicoo_tile_kernel(HiCOOArgs h, Fact
     __syncthreads();
     const uint32_t base = h.unit_br[u] << h.sb_bits;
     const bool excl = h.unit_excl[u] != 0;
+    const bool whole = excl && a.out_cols != 0;
     for (uint32_t i = tid; i < rows * GV; i += NT) {
         const uint32_t r = i / GV, c = i % GV;
         const uint32_t orow = base + r;
-        if (c >= a.R || orow >= h.nrows) continue;
+        if (orow >= h.nrows) continue;
+        float *dst = a.out + (uint64_t)orow * a.out_stride + c;
+        if (whole) {
+            if (c < a.out_cols) *dst = c < a.R ? tile[r * TP + c] : 0.f;
+            continue;
+        }
+        if (c >= a.R) continue;
         const float v = tile[r * TP + c];
         if (v == 0.f) continue;
-        float *dst = a.out + (uint64_t)orow * a.out_stride + c;
         if (excl)
             *dst = v;
         else
@@ -935,13 +945,19 @@ __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, FT>::OCC))
     __syncthreads();
     const uint32_t base = h.unit_br[u] << h.sb_bits;
     const bool excl = h.unit_excl[u] != 0;
+    const bool whole = excl && a.out_cols != 0;
     for (uint32_t i = tid; i < rows * GV; i += NT) {
         const uint32_t r = i / GV, c = i % GV;
         const uint32_t orow = base + r;
-        if (c >= a.R || orow >= h.nrows) continue;
+        if (orow >= h.nrows) continue;
+        float *dst = a.out + (uint64_t)orow * a.out_stride + c;
+        if (whole) {
+            if (c < a.out_cols) *dst = c < a.R ? tile[r * TP + c] : 0.f;
+            continue;
+        }
+        if (c >= a.R) continue;
         const float v = tile[r * TP + c];
         if (v == 0.f) continue;
-        float *dst = a.out + (uint64_t)orow * a.out_stride + c;
         if (excl)
             *dst = v;
         else
