@@ -787,6 +787,8 @@ int ensure_owned_rows(const ptiDevHiCOO *t, uint32_t mode, hipStream_t s, const 
                            c.unit_excl.as<uint32_t>(), c.nunits, owned.as<uint8_t>());
         PTI_HIP(hipGetLastError(), M);
     }
+    // Calls on other streams may read the flags as soon as they are published.
+    PTI_HIP(hipStreamSynchronize(s), M);
     c.br_owned = std::move(owned);
     return 0;
 }
