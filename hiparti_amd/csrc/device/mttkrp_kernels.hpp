@@ -875,10 +875,19 @@ __device__ __forceinline__ void stream_walk(const FactorArgs &a, const uint4 *__
 // SIMD (104-116 VGPRs); measured 3.5% faster than D = 2 on the 200M-nonzero
 // order-4 tensor at R = 64.  The CP-completion gradient walk, which also
 // holds the target row's sums, keeps D_RESID (D = 4 spills there).
+// With NRES positions read from LDS-resident copies, only the other
+// positions' rows come from memory: where those take at most 12 registers the
+// walk gathers four positions ahead as well (order 5 at four FP32 ranks per
+// lane with one or two resident factors: the CP-ALS leg's five MTTKRPs went
+// from 1.42 / 1.36 / 1.33 / 1.63 / 1.42 to 1.37 / 1.30 / 1.26 / 1.59 / 1.37 ms).
 template <int N, int V, typename FT>
 struct StreamConfig {
     static constexpr int ROW_REGS = (N - 1) * V * (int)sizeof(FT) / 4;
     static constexpr int D = ROW_REGS <= 12 ? 4 : (ROW_REGS <= 16 ? 2 : 1);
+    template <int NRES>
+    static constexpr int depth() {
+        return (N - 1 - NRES) * V * (int)sizeof(FT) / 4 <= 12 ? 4 : D;
+    }
     static constexpr int D_RESID = ROW_REGS <= 8 ? 4 : (ROW_REGS <= 16 ? 2 : 1);
     static constexpr int OCC = N > 3 ? 4 : ((int)sizeof(FT) * V >= 16 ? 5 : 6);
 };
@@ -896,7 +905,7 @@ struct StreamConfig {
 template <int N, int V, int G, typename FT, int W, int NRES = 0, bool PACK = false>
 __global__ void __launch_bounds__(W * 64, (StreamConfig<N, V, FT>::OCC))
     hicoo_stream_kernel(StreamArgs h, FactorArgs a) {
-    constexpr int D = StreamConfig<N, V, FT>::D;
+    constexpr int D = StreamConfig<N, V, FT>::template depth<NRES>();
     constexpr int NP = StreamShape<N, PACK>::NP;
     extern __shared__ float tile[];  // (1 << sb_bits) rows x TP floats [, resident factors]
     constexpr int GV = G * V;
