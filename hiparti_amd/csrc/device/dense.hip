@@ -563,7 +563,9 @@ int dense_gram(const float *A, uint64_t nrows, uint32_t R, uint32_t stride, doub
     PTI_CHECK(w.ensure(R, M), M);
     if (stride < R) PTI_FAIL(PTIERR_SHAPE_MISMATCH, M, "stride < rank");
     const uint32_t ntiles = gram_tiles(R);
-    uint32_t nsplit = cdiv(nrows, 1024);
+    // R <= 16: one 256-row stage per workgroup up to MAX_PARTS workgroups, so
+    // that factors of a few thousand rows are not left to one workgroup.
+    uint32_t nsplit = cdiv(nrows, R <= 16 ? G16ROWS : 1024);
     const uint32_t cap = MAX_PARTS / ntiles;
     if (nsplit > cap) nsplit = cap;
     if (nsplit == 0) nsplit = 1;
@@ -607,7 +609,7 @@ int dense_colstat(float *A, const float *B, uint64_t nrows, uint32_t R, uint32_t
                   bool scale, double *mul, DenseWork &w, hipStream_t s, const char *M) {
     PTI_CHECK(w.ensure(R, M), M);
     if (stride < R) PTI_FAIL(PTIERR_SHAPE_MISMATCH, M, "stride < rank");
-    uint32_t nsplit = cdiv(nrows, 1024);
+    uint32_t nsplit = cdiv(nrows, 256);   // 256 rows per workgroup up to MAX_PARTS workgroups
     if (nsplit > MAX_PARTS) nsplit = MAX_PARTS;
     if (nsplit == 0) nsplit = 1;
     const uint64_t per = (nrows + nsplit - 1) / nsplit;
